@@ -1,0 +1,68 @@
+"""Build recipe for the package's native code (gfx950 HIP library, host facade driver).
+
+Outputs, in-tree so they travel with the source tree:
+  3ddg_amd/libdg3d.so   C ABI of include/dg3d.h (HIP kernels + host setup)
+  3ddg_amd/dg3d_driver  C++ driver with the reference's Grid/Block/Element call surface
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# packed-FP32 instructions are disabled for every HIP translation unit
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", f"-I{ROOT}/include", f"-I{CSRC}"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_library(verbose_resources=False):
+    objdir = os.path.join(PKG, "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in ["dg3d_kernels.hip"]:
+        obj = os.path.join(objdir, src + ".o")
+        extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
+        _run([HIPCC, "-c", f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, *extra,
+              os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    for src in ["dg3d_capi.cpp", "dg3d_setup.cpp", "basis.cpp"]:
+        obj = os.path.join(objdir, src + ".o")
+        _run([HIPCC, "-x", "c++", "-c", *COMMON, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    out = os.path.join(PKG, "libdg3d.so")
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", *objs, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_driver():
+    host = os.path.join(CSRC, "host")
+    if not os.path.isdir(host):
+        return None
+    srcs = sorted(os.path.join(host, f) for f in os.listdir(host) if f.endswith(".cpp"))
+    if not srcs:
+        return None
+    out = os.path.join(PKG, "dg3d_driver")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT}/include", f"-I{host}", *srcs,
+          f"-L{PKG}", "-ldg3d", "-Wl,-rpath,$ORIGIN", "-ldl", "-o", out])
+    return out
+
+
+def build_all(verbose_resources=False):
+    lib = build_library(verbose_resources)
+    drv = build_driver()
+    return lib, drv
+
+
+if __name__ == "__main__":
+    build_all(verbose_resources="-v" in sys.argv)

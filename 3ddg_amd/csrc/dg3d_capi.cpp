@@ -1,0 +1,714 @@
+// C ABI of the HIP implementation (include/dg3d.h). Owns all device memory of a block:
+// state registers (the reference's Grid copies), per-element material, per-axis widths,
+// the precomputed line operator, halo face traces and the work registers of the fused step.
+#include "dg3d.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "basis.hpp"
+#include "dg3d_device.hpp"
+
+namespace {
+
+// LSRK(5,4) 2N-storage coefficients, Carpenter & Kennedy (1994), NASA TM-109112.
+const double kLsA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                        -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+const double kLsB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                        1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                        2277821191437.0 / 14882151754819.0};
+const double kLsC[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                        2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
+
+thread_local std::string g_create_error;
+
+struct Register {
+  double* d = nullptr;          // nfields * fs doubles
+  double* halo[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool halo_valid[6] = {false, false, false, false, false, false};
+};
+
+}  // namespace
+
+struct dg3d_ctx {
+  int N = 0, n = 0, np = 0;
+  int nE[3] = {0, 0, 0};
+  int periodic[3] = {0, 0, 0};
+  int64_t E = 0, fs = 0;
+  int nfields = 4;
+  int precision = DG3D_FP64, contraction = DG3D_CONTRACT_VALU, xcd = 1;
+  hipStream_t stream = nullptr;
+  std::vector<Register> regs;
+  double* work_q = nullptr;
+  void* work_du = nullptr;
+  double* work_k1 = nullptr;
+  double* work_k2 = nullptr;
+  double* d_inv_w[3] = {nullptr, nullptr, nullptr};
+  std::vector<double> h_w[3];
+  double corner[3] = {0, 0, 0};
+  double* d_mat = nullptr;
+  std::vector<double> h_mat;
+  double* d_diff = nullptr;
+  std::vector<double> h_diff;
+  double* d_opd[3] = {nullptr, nullptr, nullptr};
+  float* d_opf[3] = {nullptr, nullptr, nullptr};
+  std::vector<double> nodes, weights;
+  double* d_wq = nullptr;
+  dg3d_source src{};
+  double* d_lead = nullptr;
+  int64_t* d_probe_nodes = nullptr;
+  double* d_probe_buf = nullptr;
+  int probe_count = 0, probe_cap = 0, probe_steps = 0;
+  double* d_part = nullptr;
+  std::vector<double> h_part;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int last_launches = 0;
+  std::string err;
+
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    return code;
+  }
+  int hip_fail(hipError_t e, const char* what) {
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return DG3D_E_HIP;
+  }
+  int64_t face_elems(int face) const {
+    const int axis = 2 - face / 2;
+    const int b = (axis == 0) ? 1 : 0, c = (axis == 2) ? 1 : 2;
+    return (int64_t)nE[b] * nE[c];
+  }
+  dg3d::MeshArgs mesh() const {
+    dg3d::MeshArgs m;
+    for (int a = 0; a < 3; ++a) {
+      m.nE[a] = nE[a];
+      m.periodic[a] = periodic[a];
+      m.inv_w[a] = d_inv_w[a];
+    }
+    m.E = E;
+    m.fs = fs;
+    m.stride[0] = (int64_t)nE[1] * nE[2];
+    m.stride[1] = nE[2];
+    m.stride[2] = 1;
+    m.mat = d_mat;
+    return m;
+  }
+  dg3d::ExtArgs ext_for(int reg, double t) const {
+    dg3d::ExtArgs x;
+    for (int f = 0; f < 6; ++f) x.trace[f] = (reg >= 0 && regs[reg].halo_valid[f]) ? regs[reg].halo[f] : nullptr;
+    x.src_face = src.face;
+    x.amp = src.amplitude;
+    x.freq = src.frequency;
+    x.phase = src.phase;
+    x.t0 = src.t0;
+    x.tau = src.tau;
+    x.lead = d_lead;
+    x.t = t;
+    return x;
+  }
+  bool reg_ok(int r) const { return r >= 0 && r < (int)regs.size(); }
+};
+
+namespace {
+
+template <typename T>
+hipError_t dmalloc(T** p, size_t count) {
+  return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (count ? count : 1));
+}
+
+void free_ctx(dg3d_ctx* c) {
+  if (!c) return;
+  for (auto& r : c->regs) {
+    hipFree(r.d);
+    for (auto* h : r.halo) hipFree(h);
+  }
+  hipFree(c->work_q);
+  hipFree(c->work_du);
+  hipFree(c->work_k1);
+  hipFree(c->work_k2);
+  for (int a = 0; a < 3; ++a) {
+    hipFree(c->d_inv_w[a]);
+    hipFree(c->d_opd[a]);
+    hipFree(c->d_opf[a]);
+  }
+  hipFree(c->d_mat);
+  hipFree(c->d_diff);
+  hipFree(c->d_wq);
+  hipFree(c->d_lead);
+  hipFree(c->d_probe_nodes);
+  hipFree(c->d_probe_buf);
+  hipFree(c->d_part);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int upload_material(dg3d_ctx* c) {
+  hipError_t e = hipMemcpyAsync(c->d_mat, c->h_mat.data(), sizeof(double) * c->h_mat.size(),
+                                hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "material upload");
+  e = hipMemcpyAsync(c->d_diff, c->h_diff.data(), sizeof(double) * c->h_diff.size(), hipMemcpyHostToDevice,
+                     c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "material upload");
+  e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "material upload");
+  return DG3D_OK;
+}
+
+int ensure_work(dg3d_ctx* c, int scheme) {
+  hipError_t e = hipSuccess;
+  if (scheme == DG3D_LSRK54) {
+    if (!c->work_q) e = dmalloc(&c->work_q, (size_t)c->nfields * c->fs);
+    if (e == hipSuccess && !c->work_du) {
+      const size_t bytes = (c->precision == DG3D_MIXED ? sizeof(float) : sizeof(double)) * 4 * (size_t)c->fs;
+      e = hipMalloc(&c->work_du, bytes);
+    }
+  } else {
+    if (!c->work_k1) e = dmalloc(&c->work_k1, (size_t)4 * c->fs);
+    if (e == hipSuccess && !c->work_k2) e = dmalloc(&c->work_k2, (size_t)4 * c->fs);
+  }
+  if (e != hipSuccess) return c->fail(DG3D_E_NOMEM, std::string("work register allocation: ") + hipGetErrorString(e));
+  return DG3D_OK;
+}
+
+dg3d::RhsLaunch base_launch(const dg3d_ctx* c) {
+  dg3d::RhsLaunch L{};
+  L.n = c->n;
+  L.mesh = c->mesh();
+  L.xcd_remap = c->xcd;
+  for (int i = 0; i < 3; ++i) {
+    L.opd[i] = c->d_opd[i];
+    L.opf[i] = c->d_opf[i];
+  }
+  return L;
+}
+
+int run_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t, double** cur, double** other, bool first) {
+  dg3d::RhsLaunch L = base_launch(c);
+  L.mixed = (c->precision == DG3D_MIXED) ? 1 : 0;
+  L.mask = dg3d::kTermAll;
+  L.dt = dt;
+  if (scheme == DG3D_LSRK54) {
+    L.epilogue = dg3d::EPI_LSRK;
+    L.q_in = *cur;
+    L.q_out = *other;
+    L.du = c->work_du;
+    L.a = kLsA[stage];
+    L.b = kLsB[stage];
+    L.ext = c->ext_for(first ? 0 : -1, t + kLsC[stage] * dt);
+  } else {
+    L.epilogue = dg3d::EPI_RK33;
+    double* u0 = c->regs[0].d;
+    if (stage == 0) {
+      L.q_in = u0; L.out = c->work_k1; L.old = nullptr; L.a = 0.0; L.b = 1.0;
+      L.ext = c->ext_for(first ? 0 : -1, t);
+    } else if (stage == 1) {
+      L.q_in = c->work_k1; L.out = c->work_k2; L.old = u0; L.a = 0.75; L.b = 0.25;
+      L.ext = c->ext_for(-1, t + dt);
+    } else {
+      L.q_in = c->work_k2; L.out = u0; L.old = u0; L.a = 1.0 / 3.0; L.b = 2.0 / 3.0;
+      L.ext = c->ext_for(-1, t + 0.5 * dt);
+    }
+  }
+  hipError_t e = dg3d::launch_rhs(L, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "stage kernel launch");
+  if (scheme == DG3D_LSRK54) std::swap(*cur, *other);
+  return DG3D_OK;
+}
+
+int reduce_partials(dg3d_ctx* c, int per_block, int blocks) {
+  c->h_part.resize((size_t)per_block * blocks);
+  hipError_t e = hipMemcpyAsync(c->h_part.data(), c->d_part, sizeof(double) * c->h_part.size(),
+                                hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "reduction");
+  return DG3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dg3d_abi_version(void) { return DG3D_ABI_VERSION; }
+const char* dg3d_create_error(void) { return g_create_error.c_str(); }
+const char* dg3d_last_error(const dg3d_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
+  if (!out) return DG3D_E_INVALID;
+  *out = nullptr;
+  g_create_error.clear();
+  if (!d) { g_create_error = "null descriptor"; return DG3D_E_INVALID; }
+  if (d->order < 1 || d->order > 7) { g_create_error = "order must be 1..7"; return DG3D_E_UNSUPPORTED; }
+  if (!d->mass || !d->stiff || !d->lgl_nodes || !d->lgl_weights) {
+    g_create_error = "mass, stiff, lgl_nodes and lgl_weights are required";
+    return DG3D_E_INVALID;
+  }
+  for (int a = 0; a < 3; ++a)
+    if (d->num_elements[a] < 1) { g_create_error = "num_elements must be >= 1"; return DG3D_E_INVALID; }
+  if (d->num_registers < 1 || d->num_registers > 64) { g_create_error = "num_registers must be 1..64"; return DG3D_E_INVALID; }
+  if (d->precision != DG3D_FP64 && d->precision != DG3D_MIXED) { g_create_error = "bad precision"; return DG3D_E_INVALID; }
+  if (d->precision == DG3D_MIXED && d->contraction != DG3D_CONTRACT_VALU) {
+    g_create_error = "MFMA contractions are not built into this library";
+    return DG3D_E_UNSUPPORTED;
+  }
+  dg3d_ctx* c = new (std::nothrow) dg3d_ctx();
+  if (!c) return DG3D_E_NOMEM;
+  try {
+    c->N = d->order;
+    c->n = d->order + 1;
+    c->np = c->n * c->n * c->n;
+    for (int a = 0; a < 3; ++a) {
+      c->nE[a] = d->num_elements[a];
+      c->periodic[a] = d->periodic[a] ? 1 : 0;
+    }
+    c->E = (int64_t)c->nE[0] * c->nE[1] * c->nE[2];
+    c->fs = c->E * c->np;
+    c->nfields = d->with_q ? 7 : 4;
+    c->precision = d->precision;
+    c->contraction = d->contraction;
+    c->xcd = d->xcd_remap;
+    c->nodes.assign(d->lgl_nodes, d->lgl_nodes + c->n);
+    c->weights.assign(d->lgl_weights, d->lgl_weights + c->n);
+    dg3d::LineOperator op;
+    if (!dg3d::make_line_operator(c->n, d->mass, d->stiff, &op)) {
+      g_create_error = "mass matrix is singular";
+      free_ctx(c);
+      return DG3D_E_INVALID;
+    }
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+      g_create_error = std::string("HIP init: ") + hipGetErrorString(e);
+      free_ctx(c);
+      return DG3D_E_HIP;
+    }
+    const std::vector<double>* srcs[3] = {&op.A, &op.L0, &op.LN};
+    for (int i = 0; i < 3; ++i) {
+      const size_t cnt = srcs[i]->size();
+      std::vector<float> f(srcs[i]->begin(), srcs[i]->end());
+      if (dmalloc(&c->d_opd[i], cnt) != hipSuccess || dmalloc(&c->d_opf[i], cnt) != hipSuccess) {
+        g_create_error = "operator allocation failed";
+        free_ctx(c);
+        return DG3D_E_NOMEM;
+      }
+      hipMemcpy(c->d_opd[i], srcs[i]->data(), sizeof(double) * cnt, hipMemcpyHostToDevice);
+      hipMemcpy(c->d_opf[i], f.data(), sizeof(float) * cnt, hipMemcpyHostToDevice);
+    }
+    for (int a = 0; a < 3; ++a) {
+      c->h_w[a].assign(c->nE[a], 1.0);
+      std::vector<double> iw(c->nE[a], 1.0);
+      if (dmalloc(&c->d_inv_w[a], c->nE[a]) != hipSuccess) { g_create_error = "geometry allocation failed"; free_ctx(c); return DG3D_E_NOMEM; }
+      hipMemcpy(c->d_inv_w[a], iw.data(), sizeof(double) * iw.size(), hipMemcpyHostToDevice);
+    }
+    c->h_mat.assign((size_t)c->E * 4, 0.0);
+    for (int64_t i = 0; i < c->E; ++i) {
+      c->h_mat[i * 4 + 0] = 1500.0;
+      c->h_mat[i * 4 + 1] = 1000.0;
+    }
+    c->h_diff.assign((size_t)c->E, 0.0);
+    if (dmalloc(&c->d_mat, c->h_mat.size()) != hipSuccess || dmalloc(&c->d_diff, c->h_diff.size()) != hipSuccess ||
+        dmalloc(&c->d_wq, c->n) != hipSuccess || dmalloc(&c->d_part, 1024 * 16) != hipSuccess) {
+      g_create_error = "material allocation failed";
+      free_ctx(c);
+      return DG3D_E_NOMEM;
+    }
+    hipMemcpy(c->d_wq, c->weights.data(), sizeof(double) * c->n, hipMemcpyHostToDevice);
+    if (upload_material(c) != DG3D_OK) { g_create_error = c->err; free_ctx(c); return DG3D_E_HIP; }
+    c->regs.resize(d->num_registers);
+    for (auto& r : c->regs) {
+      if (dmalloc(&r.d, (size_t)c->nfields * c->fs) != hipSuccess) {
+        g_create_error = "register allocation failed (" + std::to_string((double)c->nfields * c->fs * 8 / 1e9) + " GB each)";
+        free_ctx(c);
+        return DG3D_E_NOMEM;
+      }
+      hipMemsetAsync(r.d, 0, sizeof(double) * c->nfields * c->fs, c->stream);
+    }
+    c->src.face = -1;
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { g_create_error = std::string("HIP: ") + hipGetErrorString(e); free_ctx(c); return DG3D_E_HIP; }
+  } catch (const std::exception& ex) {
+    g_create_error = ex.what();
+    free_ctx(c);
+    return DG3D_E_NOMEM;
+  }
+  *out = c;
+  return DG3D_OK;
+}
+
+void dg3d_destroy(dg3d_ctx* ctx) {
+  if (ctx && ctx->stream) hipStreamSynchronize(ctx->stream);
+  free_ctx(ctx);
+}
+
+int dg3d_set_geometry(dg3d_ctx* c, const double* corner, const double* wx, const double* wy, const double* wz) {
+  if (!c) return DG3D_E_INVALID;
+  const double* w[3] = {wx, wy, wz};
+  for (int a = 0; a < 3; ++a) {
+    if (corner) c->corner[a] = corner[a];
+    if (!w[a]) continue;
+    std::vector<double> iw(c->nE[a]);
+    for (int i = 0; i < c->nE[a]; ++i) {
+      if (!(w[a][i] > 0.0) || !std::isfinite(w[a][i])) return c->fail(DG3D_E_INVALID, "element widths must be positive");
+      c->h_w[a][i] = w[a][i];
+      iw[i] = 1.0 / w[a][i];
+    }
+    hipError_t e = hipMemcpy(c->d_inv_w[a], iw.data(), sizeof(double) * iw.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return c->hip_fail(e, "geometry upload");
+  }
+  return DG3D_OK;
+}
+
+int dg3d_set_material(dg3d_ctx* c, const double* c0, const double* rho0, const double* ba, const double* sigma,
+                      const double* diff) {
+  if (!c) return DG3D_E_INVALID;
+  for (int64_t i = 0; i < c->E; ++i) {
+    if (c0) c->h_mat[i * 4 + 0] = c0[i];
+    if (rho0) c->h_mat[i * 4 + 1] = rho0[i];
+    if (ba) c->h_mat[i * 4 + 2] = ba[i];
+    if (sigma) c->h_mat[i * 4 + 3] = sigma[i];
+    if (diff) c->h_diff[i] = diff[i];
+    if (!(c->h_mat[i * 4 + 0] > 0.0) || !(c->h_mat[i * 4 + 1] > 0.0))
+      return c->fail(DG3D_E_INVALID, "c0 and rho0 must be positive");
+  }
+  return upload_material(c);
+}
+
+int dg3d_set_state(dg3d_ctx* c, int reg, const double* host, int nfields) {
+  if (!c || !c->reg_ok(reg) || !host) return c ? c->fail(DG3D_E_INVALID, "set_state: bad register or pointer") : DG3D_E_INVALID;
+  if (nfields < 1 || nfields > c->nfields) return c->fail(DG3D_E_INVALID, "set_state: nfields exceeds the register");
+  hipError_t e = hipMemcpy(c->regs[reg].d, host, sizeof(double) * nfields * c->fs, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return c->hip_fail(e, "set_state");
+  return DG3D_OK;
+}
+
+int dg3d_get_state(dg3d_ctx* c, int reg, double* host, int nfields) {
+  if (!c || !c->reg_ok(reg) || !host) return c ? c->fail(DG3D_E_INVALID, "get_state: bad register or pointer") : DG3D_E_INVALID;
+  if (nfields < 1 || nfields > c->nfields) return c->fail(DG3D_E_INVALID, "get_state: nfields exceeds the register");
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(host, c->regs[reg].d, sizeof(double) * nfields * c->fs, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return c->hip_fail(e, "get_state");
+  return DG3D_OK;
+}
+
+int dg3d_set_halo(dg3d_ctx* c, int reg, int face, const double* trace) {
+  if (!c || !c->reg_ok(reg) || face < 0 || face > 5) return c ? c->fail(DG3D_E_INVALID, "set_halo: bad register or face") : DG3D_E_INVALID;
+  Register& r = c->regs[reg];
+  if (!trace) {
+    r.halo_valid[face] = false;
+    return DG3D_OK;
+  }
+  const size_t cnt = (size_t)c->face_elems(face) * c->n * c->n * 3;
+  if (!r.halo[face] && dmalloc(&r.halo[face], cnt) != hipSuccess) return c->fail(DG3D_E_NOMEM, "halo allocation failed");
+  hipError_t e = hipMemcpy(r.halo[face], trace, sizeof(double) * cnt, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return c->hip_fail(e, "set_halo");
+  r.halo_valid[face] = true;
+  return DG3D_OK;
+}
+
+int dg3d_zero(dg3d_ctx* c, int reg) {
+  if (!c || !c->reg_ok(reg)) return c ? c->fail(DG3D_E_INVALID, "zero: bad register") : DG3D_E_INVALID;
+  Register& r = c->regs[reg];
+  hipError_t e = hipMemsetAsync(r.d, 0, sizeof(double) * c->nfields * c->fs, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "zero");
+  for (bool& v : r.halo_valid) v = false;
+  return DG3D_OK;
+}
+
+int dg3d_copy(dg3d_ctx* c, int dst, int src) {
+  if (!c || !c->reg_ok(dst) || !c->reg_ok(src)) return c ? c->fail(DG3D_E_INVALID, "copy: bad register") : DG3D_E_INVALID;
+  if (dst == src) return DG3D_OK;
+  Register& d = c->regs[dst];
+  const Register& s = c->regs[src];
+  hipError_t e = hipMemcpyAsync(d.d, s.d, sizeof(double) * c->nfields * c->fs, hipMemcpyDeviceToDevice, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "copy");
+  for (int f = 0; f < 6; ++f) {
+    d.halo_valid[f] = s.halo_valid[f];
+    if (!s.halo_valid[f]) continue;
+    const size_t cnt = (size_t)c->face_elems(f) * c->n * c->n * 3;
+    if (!d.halo[f] && dmalloc(&d.halo[f], cnt) != hipSuccess) return c->fail(DG3D_E_NOMEM, "halo allocation failed");
+    e = hipMemcpyAsync(d.halo[f], s.halo[f], sizeof(double) * cnt, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "copy halo");
+  }
+  return DG3D_OK;
+}
+
+int dg3d_axpby(dg3d_ctx* c, int out, double self_scalar, int in, double other_scalar) {
+  if (!c || !c->reg_ok(out) || !c->reg_ok(in)) return c ? c->fail(DG3D_E_INVALID, "axpby: bad register") : DG3D_E_INVALID;
+  hipError_t e = dg3d::launch_axpby(c->regs[out].d, self_scalar, c->regs[in].d, other_scalar,
+                                    (int64_t)c->nfields * c->fs, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "axpby");
+  return DG3D_OK;
+}
+
+int dg3d_operate(dg3d_ctx* c, int op, int axis, int mode, int in_reg, int out_reg) {
+  if (!c || !c->reg_ok(in_reg) || !c->reg_ok(out_reg)) return c ? c->fail(DG3D_E_INVALID, "operate: bad register") : DG3D_E_INVALID;
+  if (op == DG3D_OP_UFROMV || op == DG3D_OP_SPONGE) {
+    if (op == DG3D_OP_UFROMV) {
+      if (axis < 0 || axis > 2 || mode < 0 || mode > 1) return c->fail(DG3D_E_INVALID, "operate: axis 0..2, mode 0..1");
+      if (in_reg == out_reg) return c->fail(DG3D_E_INVALID, "operate: UFromV writes a register other than the one it reads");
+    }
+    dg3d::RhsLaunch L = base_launch(c);
+    L.mixed = 0;
+    L.epilogue = dg3d::EPI_ACCUM;
+    L.mask = (op == DG3D_OP_SPONGE) ? dg3d::kTermSponge : (1u << (2 * axis + mode));
+    L.q_in = c->regs[in_reg].d;
+    L.out = c->regs[out_reg].d;
+    L.ext = c->ext_for(in_reg, 0.0);
+    hipError_t e = dg3d::launch_rhs(L, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "operate");
+    return DG3D_OK;
+  }
+  if (op == DG3D_OP_QFROMDRHO || op == DG3D_OP_DRHOFROMQ) {
+    if (c->nfields < 7) return c->fail(DG3D_E_INVALID, "operate: LDG operators need with_q");
+    if (axis < 0 || axis > 2) return c->fail(DG3D_E_INVALID, "operate: axis 0..2");
+    if (op == DG3D_OP_DRHOFROMQ && in_reg == out_reg)
+      return c->fail(DG3D_E_INVALID, "operate: DRhoFromQ writes a register other than the one it reads");
+    dg3d::LdgLaunch L{};
+    L.n = c->n;
+    L.which = (op == DG3D_OP_QFROMDRHO) ? 0 : 1;
+    L.axis = axis;
+    L.mesh = c->mesh();
+    L.ext = c->ext_for(in_reg, 0.0);
+    L.diff = c->d_diff;
+    for (int i = 0; i < 3; ++i) L.opd[i] = c->d_opd[i];
+    L.in = c->regs[in_reg].d;
+    L.out = c->regs[out_reg].d;
+    hipError_t e = dg3d::launch_ldg(L, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "operate");
+    return DG3D_OK;
+  }
+  return c->fail(DG3D_E_INVALID, "operate: unknown operator");
+}
+
+int dg3d_halo(dg3d_ctx* c, int reg) {
+  if (!c || !c->reg_ok(reg)) return c ? c->fail(DG3D_E_INVALID, "halo: bad register") : DG3D_E_INVALID;
+  return DG3D_OK;
+}
+
+int dg3d_set_source(dg3d_ctx* c, const dg3d_source* s) {
+  if (!c) return DG3D_E_INVALID;
+  hipFree(c->d_lead);
+  c->d_lead = nullptr;
+  if (!s || s->face < 0) {
+    c->src = dg3d_source{};
+    c->src.face = -1;
+    return DG3D_OK;
+  }
+  if (s->face > 5) return c->fail(DG3D_E_INVALID, "source face must be 0..5");
+  if (c->periodic[2 - s->face / 2]) return c->fail(DG3D_E_INVALID, "source on a periodic face");
+  if (!(s->tau > 0.0)) return c->fail(DG3D_E_INVALID, "source tau must be positive");
+  c->src = *s;
+  c->src.lead = nullptr;
+  if (s->lead) {
+    const size_t cnt = (size_t)c->face_elems(s->face) * c->n * c->n;
+    if (dmalloc(&c->d_lead, cnt) != hipSuccess) return c->fail(DG3D_E_NOMEM, "source allocation failed");
+    hipError_t e = hipMemcpy(c->d_lead, s->lead, sizeof(double) * cnt, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return c->hip_fail(e, "source upload");
+  }
+  return DG3D_OK;
+}
+
+int dg3d_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t) {
+  if (!c) return DG3D_E_INVALID;
+  const int ns = (scheme == DG3D_LSRK54) ? 5 : 3;
+  if ((scheme != DG3D_LSRK54 && scheme != DG3D_RK33) || stage < 0 || stage >= ns)
+    return c->fail(DG3D_E_INVALID, "stage: bad scheme or stage");
+  int rc = ensure_work(c, scheme);
+  if (rc) return rc;
+  double* cur = c->regs[0].d;
+  double* other = c->work_q;
+  const bool first = (stage == 0);
+  rc = run_stage(c, scheme, stage, dt, t, &cur, &other, first);
+  if (rc) return rc;
+  if (scheme == DG3D_LSRK54) {
+    // the new state becomes register 0; the old buffer becomes the work register
+    if (c->nfields > 4) {
+      hipMemcpyAsync(cur + 4 * c->fs, c->regs[0].d + 4 * c->fs, sizeof(double) * 3 * c->fs, hipMemcpyDeviceToDevice,
+                     c->stream);
+    }
+    c->work_q = c->regs[0].d;
+    c->regs[0].d = cur;
+  }
+  if (first)
+    for (bool& v : c->regs[0].halo_valid) v = false;
+  return DG3D_OK;
+}
+
+int dg3d_step(dg3d_ctx* c, int scheme, double dt, int nsteps, double* t) {
+  if (!c || !t) return DG3D_E_INVALID;
+  if (scheme != DG3D_LSRK54 && scheme != DG3D_RK33) return c->fail(DG3D_E_INVALID, "step: unknown scheme");
+  if (nsteps < 0) return c->fail(DG3D_E_INVALID, "step: nsteps < 0");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return c->fail(DG3D_E_INVALID, "step: dt must be positive");
+  int rc = ensure_work(c, scheme);
+  if (rc) return rc;
+  if (nsteps == 0) return DG3D_OK;
+  const int ns = (scheme == DG3D_LSRK54) ? 5 : 3;
+  double* cur = c->regs[0].d;
+  double* other = c->work_q;
+  double tt = *t;
+  hipEventRecord(c->ev0, c->stream);
+  bool first = true;
+  for (int s = 0; s < nsteps; ++s) {
+    for (int st = 0; st < ns; ++st) {
+      rc = run_stage(c, scheme, st, dt, tt, &cur, &other, first);
+      if (rc) return rc;
+      if (first) {
+        for (bool& v : c->regs[0].halo_valid) v = false;
+        first = false;
+      }
+    }
+    tt += dt;
+    if (c->probe_count > 0 && c->probe_steps < c->probe_cap) {
+      hipError_t e = dg3d::launch_probe(cur, c->mesh(), c->n, c->d_probe_nodes, c->probe_count,
+                                        c->d_probe_buf + (size_t)c->probe_steps * c->probe_count, c->stream);
+      if (e != hipSuccess) return c->hip_fail(e, "probe");
+      ++c->probe_steps;
+    }
+  }
+  hipEventRecord(c->ev1, c->stream);
+  if (scheme == DG3D_LSRK54 && cur != c->regs[0].d) {
+    if (c->nfields > 4)
+      hipMemcpyAsync(cur + 4 * c->fs, c->regs[0].d + 4 * c->fs, sizeof(double) * 3 * c->fs, hipMemcpyDeviceToDevice,
+                     c->stream);
+    c->work_q = c->regs[0].d;
+    c->regs[0].d = cur;
+  }
+  hipError_t e = hipEventSynchronize(c->ev1);
+  if (e != hipSuccess) return c->hip_fail(e, "step");
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->last_ms = ms;
+  c->last_launches = nsteps * ns;
+  *t = tt;
+  return DG3D_OK;
+}
+
+int dg3d_last_step_timing(dg3d_ctx* c, double* avg_stage_ms, int* launches) {
+  if (!c) return DG3D_E_INVALID;
+  if (avg_stage_ms) *avg_stage_ms = c->last_launches ? c->last_ms / c->last_launches : 0.0;
+  if (launches) *launches = c->last_launches;
+  return DG3D_OK;
+}
+
+int dg3d_cfl_dt(dg3d_ctx* c, double cfl, double* dt) {
+  if (!c || !dt) return DG3D_E_INVALID;
+  double dxi = 2.0;
+  for (int i = 0; i + 1 < c->n; ++i) dxi = std::min(dxi, c->nodes[i + 1] - c->nodes[i]);
+  hipError_t e = dg3d::launch_cfl(c->mesh(), 0.5 * dxi, c->d_part, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "cfl");
+  const int blocks = dg3d::reduce_blocks(c->E);
+  int rc = reduce_partials(c, 1, blocks);
+  if (rc) return rc;
+  double m = 1e300;
+  for (int b = 0; b < blocks; ++b) m = std::min(m, c->h_part[b]);
+  *dt = cfl * m;
+  return DG3D_OK;
+}
+
+int dg3d_reduce(dg3d_ctx* c, int kind, int reg, int reg2, double* out) {
+  if (!c || !c->reg_ok(reg) || !out) return c ? c->fail(DG3D_E_INVALID, "reduce: bad register") : DG3D_E_INVALID;
+  hipError_t e;
+  const int blocks = dg3d::reduce_blocks(c->fs);
+  if (kind == DG3D_REDUCE_MAXABS) {
+    e = dg3d::launch_maxabs(c->regs[reg].d, c->fs, 4, c->d_part, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "reduce");
+    int rc = reduce_partials(c, 4, blocks);
+    if (rc) return rc;
+    for (int f = 0; f < 4; ++f) {
+      double m = 0.0;
+      for (int b = 0; b < blocks; ++b) {
+        const double v = c->h_part[b * 4 + f];
+        if (std::isnan(v) || std::isnan(m)) m = std::nan("");
+        else m = std::max(m, v);
+      }
+      out[f] = m;
+    }
+    return DG3D_OK;
+  }
+  if (kind == DG3D_REDUCE_ENERGY) {
+    e = dg3d::launch_energy(c->regs[reg].d, c->mesh(), c->n, c->d_wq, c->d_part, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "reduce");
+    int rc = reduce_partials(c, 2, blocks);
+    if (rc) return rc;
+    out[0] = out[1] = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+      out[0] += c->h_part[b * 2];
+      out[1] += c->h_part[b * 2 + 1];
+    }
+    if (!std::isfinite(out[0])) return c->fail(DG3D_E_NAN, "energy is not finite");
+    return DG3D_OK;
+  }
+  if (kind == DG3D_REDUCE_RELL2 || kind == DG3D_REDUCE_DIFFL2) {
+    if (!c->reg_ok(reg2)) return c->fail(DG3D_E_INVALID, "reduce: bad second register");
+    e = dg3d::launch_l2diff(c->regs[reg].d, c->regs[reg2].d, c->fs, 4, c->d_part, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "reduce");
+    int rc = reduce_partials(c, 8, blocks);
+    if (rc) return rc;
+    double d[4] = {0, 0, 0, 0}, r[4] = {0, 0, 0, 0};
+    for (int b = 0; b < blocks; ++b)
+      for (int f = 0; f < 4; ++f) {
+        d[f] += c->h_part[(b * 4 + f) * 2];
+        r[f] += c->h_part[(b * 4 + f) * 2 + 1];
+      }
+    for (int f = 0; f < 4; ++f) {
+      if (kind == DG3D_REDUCE_RELL2) out[f] = (r[f] > 0.0) ? std::sqrt(d[f] / r[f]) : std::sqrt(d[f]);
+      else {
+        out[f] = std::sqrt(d[f]);
+        out[4 + f] = std::sqrt(r[f]);
+      }
+    }
+    return DG3D_OK;
+  }
+  return c->fail(DG3D_E_INVALID, "reduce: unknown kind");
+}
+
+int dg3d_set_probes(dg3d_ctx* c, int count, const int64_t* nodes, int capacity_steps) {
+  if (!c || count < 0 || capacity_steps < 0) return DG3D_E_INVALID;
+  hipFree(c->d_probe_nodes);
+  hipFree(c->d_probe_buf);
+  c->d_probe_nodes = nullptr;
+  c->d_probe_buf = nullptr;
+  c->probe_count = 0;
+  c->probe_cap = 0;
+  c->probe_steps = 0;
+  if (count == 0) return DG3D_OK;
+  for (int i = 0; i < count; ++i)
+    if (nodes[i] < 0 || nodes[i] >= c->fs) return c->fail(DG3D_E_INVALID, "probe node out of range");
+  if (dmalloc(&c->d_probe_nodes, count) != hipSuccess ||
+      dmalloc(&c->d_probe_buf, (size_t)count * std::max(capacity_steps, 1)) != hipSuccess)
+    return c->fail(DG3D_E_NOMEM, "probe allocation failed");
+  hipMemcpy(c->d_probe_nodes, nodes, sizeof(int64_t) * count, hipMemcpyHostToDevice);
+  c->probe_count = count;
+  c->probe_cap = capacity_steps;
+  return DG3D_OK;
+}
+
+int dg3d_get_probes(dg3d_ctx* c, double* out, int* steps) {
+  if (!c) return DG3D_E_INVALID;
+  if (steps) *steps = c->probe_steps;
+  if (out && c->probe_steps > 0) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpy(out, c->d_probe_buf, sizeof(double) * c->probe_count * c->probe_steps, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->hip_fail(e, "get_probes");
+  }
+  return DG3D_OK;
+}
+
+int dg3d_sync(dg3d_ctx* c) {
+  if (!c) return DG3D_E_INVALID;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "sync");
+  return DG3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// Device-side parameter blocks and launcher declarations shared by the C ABI (dg3d_capi.cpp)
+// and the kernels (dg3d_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace dg3d {
+
+constexpr int kMaxN = 8;  // n = N+1 <= 8 (orders 1..7)
+
+// Term mask bits of the RHS kernel: UFromV(axis, mode) = bit (2*axis + mode), sponge = bit 6.
+constexpr unsigned kTermSponge = 1u << 6;
+constexpr unsigned kTermAll = 0x7Fu;
+
+enum Epilogue : int {
+  EPI_ACCUM = 0,  // out += rhs over the masked fields (Block::Operate* into a register)
+  EPI_LSRK = 1,   // du = a du + dt rhs; q_out = q + b du
+  EPI_RK33 = 2,   // out = (rhs dt beta + q beta) + old alpha
+};
+
+struct MeshArgs {
+  int nE[3];
+  int periodic[3];
+  int64_t E;          // interior elements
+  int64_t fs;         // field stride in a register = E * n^3
+  int64_t stride[3];  // element index stride per axis: e = (i*nE1 + j)*nE2 + k
+  const double* inv_w[3];  // 1/width per element index along each axis
+  const double* mat;       // [E][4] = c0, rho0, B/A, sigma
+};
+
+struct ExtArgs {
+  const double* trace[6];  // halo face traces [faceElem][n*n][3] (rho', u_axis, Q_axis) or null
+  int src_face;            // -1 none
+  double amp, freq, phase, t0, tau;
+  const double* lead;      // [faceElem][n*n] or null
+  double t;                // stage time
+};
+
+template <typename CT>
+struct OpArgs {
+  const CT* A;   // n*n
+  const CT* L0;  // n
+  const CT* LN;  // n
+};
+
+struct RhsLaunch {
+  int n;            // nodes per axis
+  int mixed;        // 0: FP64 compute, 1: FP32 compute
+  int epilogue;     // Epilogue
+  unsigned mask;    // term mask
+  int xcd_remap;
+  MeshArgs mesh;
+  ExtArgs ext;
+  const double* opd[3];  // A, L0, LN in FP64
+  const float* opf[3];   // A, L0, LN in FP32
+  const double* q_in;    // state read (and neighbour traces)
+  double* out;           // EPI_ACCUM / EPI_RK33 output register
+  const double* old;     // EPI_RK33 old register (alpha != 0)
+  double* q_out;         // EPI_LSRK new state
+  void* du;              // EPI_LSRK register (double or float per `mixed`)
+  double a, b, dt;       // LSRK: a, b, dt; RK33: alpha, beta, dt
+};
+
+hipError_t launch_rhs(const RhsLaunch& L, hipStream_t s);
+
+// LDG operators (Element.cpp:623-758) for one axis, FP64.
+struct LdgLaunch {
+  int n;
+  int which;  // 0: QFromDRho (writes Q_axis of out), 1: DRhoFromQ (adds to rho' of out)
+  int axis;
+  MeshArgs mesh;
+  ExtArgs ext;
+  const double* diff;  // per element D
+  const double* opd[3];
+  const double* in;    // register read (7 fields)
+  double* out;         // register written
+};
+hipError_t launch_ldg(const LdgLaunch& L, hipStream_t s);
+
+// out = out*a + in*b over nfields fields of length fs.
+hipError_t launch_axpby(double* out, double a, const double* in, double b, int64_t count, hipStream_t s);
+
+// Reductions: partial results per block into `partials` (size returned by reduce_blocks).
+int reduce_blocks(int64_t count);
+hipError_t launch_maxabs(const double* reg, int64_t fs, int nfields, double* partials, hipStream_t s);
+hipError_t launch_l2diff(const double* a, const double* b, int64_t fs, int nfields, double* partials,
+                         hipStream_t s);
+hipError_t launch_energy(const double* reg, const MeshArgs& m, int n, const double* wq, double* partials,
+                         hipStream_t s);
+hipError_t launch_cfl(const MeshArgs& m, double dxi_half, double* partials, hipStream_t s);
+hipError_t launch_probe(const double* reg, const MeshArgs& m, int n, const int64_t* nodes, int count,
+                        double* out, hipStream_t s);
+
+}  // namespace dg3d

@@ -1,0 +1,203 @@
+/*
+ * dg3d.h — C ABI of the MI355X (gfx950) implementation of 3DDG's per-RK-stage
+ * right-hand side and update for the first-order Westervelt-like acoustic system
+ * (rho', u, v, w) on tensor-product LGL hexahedra.
+ *
+ * The reference has no FFI layer: its call surface is C++ methods. Each entry point
+ * below names the reference method it replaces (file:line in drewalan/3DDG cppsrc/).
+ * A C++ host facade (3ddg_amd/csrc/host/) with the reference's Grid/Block/Element
+ * names forwards to these functions; Python binds them with ctypes.
+ *
+ * Conventions
+ *  - Every function returns int status: DG3D_OK (0) or a negative DG3D_E_* code; the
+ *    message is available from dg3d_last_error(ctx). No C++ exception crosses the ABI.
+ *  - Host arrays are borrowed for the duration of the call only.
+ *  - A context is used from one host thread at a time. Calls are ordered on the
+ *    context's HIP stream; dg3d_get_state, dg3d_reduce and dg3d_sync synchronise.
+ *  - Host state layout (the reference's mElements[i][j][k].mNodes[a][b][c] order):
+ *      field-major: rho', u, v, w [, Qx, Qy, Qz];
+ *      within a field: element (i,j,k) with k fastest, then node (a,b,c) with c fastest.
+ *    i/a run along x, j/b along y, k/c along z (Element.cpp:325-348).
+ *  - Face/direction numbering is the reference's neighbour order (Element.h:144):
+ *      0 = k-1 (-z), 1 = k+1 (+z), 2 = j-1 (-y), 3 = j+1 (+y), 4 = i-1 (-x), 5 = i+1 (+x).
+ */
+#ifndef DG3D_H
+#define DG3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DG3D_ABI_VERSION 1
+
+/* status codes */
+#define DG3D_OK 0
+#define DG3D_E_INVALID (-1)     /* bad argument or call order */
+#define DG3D_E_HIP (-2)         /* HIP runtime error */
+#define DG3D_E_NOMEM (-3)       /* device allocation failed */
+#define DG3D_E_NAN (-4)         /* NaN/Inf detected by a checked reduction */
+#define DG3D_E_UNSUPPORTED (-5) /* order/precision/option not built */
+#define DG3D_E_IO (-6)          /* file missing, unreadable or malformed */
+
+/* precision of the fused step (dg3d_step) */
+#define DG3D_FP64 0  /* FP64 state, FP64 RHS: the drop-in */
+#define DG3D_MIXED 1 /* FP64 state, FP32 RHS and FP32 LSRK register */
+
+/* derivative contraction used by the MIXED step */
+#define DG3D_CONTRACT_VALU 0      /* FP32 FMA on the vector ALUs */
+#define DG3D_CONTRACT_MFMA_F32 1  /* v_mfma_f32_16x16x4_f32 */
+#define DG3D_CONTRACT_MFMA_BF16 2 /* split-BF16 (3 pieces per operand), FP32 accumulate */
+
+/* time schemes */
+#define DG3D_RK33 0   /* Shu-Osher SSP-RK3 as Grid::RK33 (Grid.cpp:642-902) */
+#define DG3D_LSRK54 1 /* Carpenter-Kennedy 5-stage 4th-order 2N-storage */
+
+/* operators of dg3d_operate (Block.h:49-52) */
+#define DG3D_OP_UFROMV 0    /* Block::OperateUFromV(axis, mode, old) */
+#define DG3D_OP_SPONGE 1    /* Block::OperateSponge(0, 0, old) */
+#define DG3D_OP_QFROMDRHO 2 /* Block::OperateQFromDRho(axis, 0, old) */
+#define DG3D_OP_DRHOFROMQ 3 /* Block::OperateDRhoFromQ(axis, 0, old) */
+
+/* reductions of dg3d_reduce */
+#define DG3D_REDUCE_MAXABS 0 /* out[0..3] = max|rho'|,|u|,|v|,|w|; NaN-sticky like HDF5Functions.cpp:201 */
+#define DG3D_REDUCE_ENERGY 1 /* out[0] = acoustic energy (LGL quadrature), out[1] = int rho' dV */
+#define DG3D_REDUCE_RELL2 2  /* out[f] = ||reg - reg2||_2 / ||reg2||_2 per field f=0..3 */
+#define DG3D_REDUCE_DIFFL2 3 /* out[f] = ||reg - reg2||_2, out[4+f] = ||reg2||_2 */
+
+typedef struct dg3d_ctx dg3d_ctx;
+
+typedef struct dg3d_desc {
+  int order;               /* N (1..7): n = N+1 LGL nodes per axis */
+  int num_elements[3];     /* interior elements per axis of the block (x, y, z) */
+  int periodic[3];         /* BC_IS_PERIODIC (driver.cpp:170) */
+  int precision;           /* DG3D_FP64 or DG3D_MIXED (for dg3d_step) */
+  int contraction;         /* DG3D_CONTRACT_* (MIXED only) */
+  int num_registers;       /* state registers (Grid copies); >= 1 */
+  int with_q;              /* allocate the LDG fields Qx, Qy, Qz (EVALUATE_DIFFUSION) */
+  int xcd_remap;           /* map workgroups so each XCD streams a contiguous slab (1 = on) */
+  const double* mass;      /* n*n row-major unit-interval mass M (matrices.txt, Grid.cpp:979) */
+  const double* stiff;     /* n*n row-major weak stiffness S (matrices.txt, Grid.cpp:980) */
+  const double* lgl_nodes; /* n LGL nodes on [-1,1] (hardcoded_coeff.txt, Basis.cpp:142-153) */
+  const double* lgl_weights; /* n LGL weights (hardcoded_coeff.txt, Basis.cpp:154-165) */
+} dg3d_desc;
+
+/* Exterior forcing on one block face: the halo state of that face becomes
+ * p = s(t + lead), u_normal = s/(rho0 c) (entering the block), rho' = s/c^2, with
+ * s(t) = amplitude * sin(2 pi f t + phase) * exp(-((t - t0)/tau)^2).
+ * No counterpart in the reference (it has initial conditions only). */
+typedef struct dg3d_source {
+  int face;            /* direction 0..5, or -1 for none */
+  double amplitude;    /* Pa */
+  double frequency;    /* Hz */
+  double phase;        /* rad */
+  double t0;           /* envelope centre, s */
+  double tau;          /* envelope width, s */
+  const double* lead;  /* per face node time lead in s [faceElems][n*n], or NULL */
+} dg3d_source;
+
+/* ---- lifecycle ------------------------------------------------------------ */
+
+/* Replaces Grid::Grid + Grid::readMatrices (Grid.cpp:176-375, 920-989): allocates the
+ * block's device registers and precomputes the line operator from M and S. */
+int dg3d_create(const dg3d_desc* desc, dg3d_ctx** out);
+void dg3d_destroy(dg3d_ctx* ctx);
+const char* dg3d_last_error(const dg3d_ctx* ctx);
+/* Library-level message for a failed dg3d_create (ctx not available). */
+const char* dg3d_create_error(void);
+int dg3d_abi_version(void);
+
+/* ---- setup ---------------------------------------------------------------- */
+
+/* Element geometry (Element.cpp:278-336, Block.cpp:172-210). corner = STARTING_CORNER;
+ * widths per axis (nE[a] values each): the reference uses one ELEMENT_WIDTH per axis
+ * (driver.cpp:161); per-element values along an axis give stretched meshes. */
+int dg3d_set_geometry(dg3d_ctx* ctx, const double* corner, const double* wx, const double* wy,
+                      const double* wz);
+
+/* Per-element material and sponge (Element.h:146-151; Block.cpp:175-233 for sigma).
+ * Arrays of E values in element order; NULL leaves the current values. */
+int dg3d_set_material(dg3d_ctx* ctx, const double* c0, const double* rho0, const double* b_over_a,
+                      const double* sigma, const double* diffusion);
+
+/* ---- registers (Grid copies) ---------------------------------------------- */
+
+/* Write / read a register's interior state (host layout above; nfields 4 or 7). */
+int dg3d_set_state(dg3d_ctx* ctx, int reg, const double* host, int nfields);
+int dg3d_get_state(dg3d_ctx* ctx, int reg, double* host, int nfields);
+
+/* Halo of a register on one face: trace[faceElem][n*n][3] = (rho', u_normal, Q_normal) of
+ * the face node layer of the halo element (what Element::OperateUFromV reads through
+ * neighbors[dir], Element.cpp:531-600). faceElem enumerates the face's elements in (i,j,k)
+ * order with the face axis left out; face nodes likewise. NULL = zero state. */
+int dg3d_set_halo(dg3d_ctx* ctx, int reg, int face, const double* trace);
+
+/* Block::setZero (Block.cpp:297-310): zero state and halo. */
+int dg3d_zero(dg3d_ctx* ctx, int reg);
+/* Grid copy constructor (Grid.cpp:78-174): dst = src, halo included. */
+int dg3d_copy(dg3d_ctx* ctx, int dst, int src);
+/* Block::selfWeightedAdd (Block.cpp:327-340): out = out*self_scalar + in*other_scalar
+ * over the interior, all allocated fields. */
+int dg3d_axpby(dg3d_ctx* ctx, int out, double self_scalar, int in, double other_scalar);
+/* Block::Operate* (Block.cpp:343-403): out_reg accumulates op(in_reg) for (axis, mode). */
+int dg3d_operate(dg3d_ctx* ctx, int op, int axis, int mode, int in_reg, int out_reg);
+/* Grid::communicate (Grid.cpp:487-565). One block on one GPU: periodic faces read the
+ * opposite interior directly, so no data moves; non-periodic halos keep their state. */
+int dg3d_halo(dg3d_ctx* ctx, int reg);
+
+/* ---- fused path ----------------------------------------------------------- */
+
+int dg3d_set_source(dg3d_ctx* ctx, const dg3d_source* src);
+
+/* Advance register 0 by nsteps steps of the given scheme (one fused kernel per stage).
+ * Replaces the driver's step loop over Grid::RK33 (driver.cpp:256-277). *t is the time of
+ * register 0 on entry and is advanced by nsteps*dt. The halo of register 0 (set by
+ * dg3d_set_halo) is read by the first stage only, as the reference reads the initial
+ * grid's halo on step 1 stage 1 and zeroed registers afterwards (Grid.cpp:648-650). */
+int dg3d_step(dg3d_ctx* ctx, int scheme, double dt, int nsteps, double* t);
+
+/* One stage of a scheme on register 0 (stage = 0..stages-1), for stage-level tests. */
+int dg3d_stage(dg3d_ctx* ctx, int scheme, int stage, double dt, double t);
+
+/* CFL time step in FP64 on the device: dt = cfl / max_e( c(e) * sum_a 2 / (w_a(e) * dxi_min) ),
+ * dxi_min the smallest LGL node gap on [-1,1] (the d-dimensional node-spacing CFL). */
+int dg3d_cfl_dt(dg3d_ctx* ctx, double cfl, double* dt);
+
+/* Reductions over a register (see DG3D_REDUCE_*); out holds up to 8 doubles. */
+int dg3d_reduce(dg3d_ctx* ctx, int kind, int reg, int reg2, double* out);
+
+/* Probe nodes: after every step of dg3d_step the pressure P(rho') at each probe node
+ * (global node index into a field) is recorded. */
+int dg3d_set_probes(dg3d_ctx* ctx, int count, const int64_t* node_index, int capacity_steps);
+int dg3d_get_probes(dg3d_ctx* ctx, double* out, int* steps_recorded);
+
+int dg3d_sync(dg3d_ctx* ctx);
+/* Average device time of the last dg3d_step's stage kernels in ms (HIP events on the
+ * context stream), and the number of stage launches it averaged. */
+int dg3d_last_step_timing(dg3d_ctx* ctx, double* avg_stage_ms, int* launches);
+
+/* ---- host-side setup helpers (no device work) ------------------------------ */
+
+/* LGL basis for order N generated in FP64 (replaces the missing hardcoded_coeff*.txt and
+ * matrices.txt inputs of Basis.cpp:74-237 and Grid.cpp:920-989). Arrays of n or n*n. */
+int dg3d_basis_generate(int order, double* nodes, double* weights, double* mass, double* stiff,
+                        double* dpsi);
+/* Write hardcoded_coeff.txt, hardcoded_coeff2.txt and matrices.txt into dir. */
+int dg3d_basis_write_files(const char* dir, int max_order);
+/* Read order N from the reference's files (Basis.cpp:130-171, Grid.cpp:940-989). */
+int dg3d_basis_read_files(const char* coeff_path, const char* matrices_path, int order,
+                          double* nodes, double* weights, double* mass, double* stiff);
+
+/* Sponge sigma per element (Block.cpp:163-233): the block at block_pos among num_blocks,
+ * each with nE elements per axis; widths are the element widths per axis; ramp 0 = linear
+ * max(0,1-d)*sigma_max (the reference), 1 = quadratic max(0,1-d)^2*sigma_max. */
+int dg3d_sponge_sigma(const int* num_elements, const int* num_blocks, const int* block_pos,
+                      const double* elem_width, const int* sponge_width, double sigma_max,
+                      int ramp, double* sigma);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DG3D_H */
