@@ -98,6 +98,8 @@ struct dg3d_ctx {
     m.stride[1] = nE[2];
     m.stride[2] = 1;
     m.mat = d_mat;
+    m.div12 = dg3d::make_div32((uint32_t)(nE[1] * nE[2]));
+    m.div2 = dg3d::make_div32((uint32_t)nE[2]);
     return m;
   }
   dg3d::ExtArgs ext_for(int reg, double t) const {
@@ -152,8 +154,17 @@ void free_ctx(dg3d_ctx* c) {
 }
 
 int upload_material(dg3d_ctx* c) {
-  hipError_t e = hipMemcpyAsync(c->d_mat, c->h_mat.data(), sizeof(double) * c->h_mat.size(),
-                                hipMemcpyHostToDevice, c->stream);
+  std::vector<double> rec((size_t)c->E * dg3d::kMatStride, 0.0);
+  for (int64_t i = 0; i < c->E; ++i) {
+    double* r = &rec[(size_t)i * dg3d::kMatStride];
+    r[dg3d::MAT_C] = c->h_mat[i * 4 + 0];
+    r[dg3d::MAT_RHO0] = c->h_mat[i * 4 + 1];
+    r[dg3d::MAT_BA] = c->h_mat[i * 4 + 2];
+    r[dg3d::MAT_SIGMA] = c->h_mat[i * 4 + 3];
+    r[dg3d::MAT_IRHO0] = 1.0 / c->h_mat[i * 4 + 1];
+  }
+  hipError_t e = hipMemcpyAsync(c->d_mat, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice,
+                                c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "material upload");
   e = hipMemcpyAsync(c->d_diff, c->h_diff.data(), sizeof(double) * c->h_diff.size(), hipMemcpyHostToDevice,
                      c->stream);
@@ -253,6 +264,10 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
   }
   for (int a = 0; a < 3; ++a)
     if (d->num_elements[a] < 1) { g_create_error = "num_elements must be >= 1"; return DG3D_E_INVALID; }
+  if ((int64_t)d->num_elements[0] * d->num_elements[1] * d->num_elements[2] >= (1ll << 31)) {
+    g_create_error = "more than 2^31 elements in one block";
+    return DG3D_E_INVALID;
+  }
   if (d->num_registers < 1 || d->num_registers > 64) { g_create_error = "num_registers must be 1..64"; return DG3D_E_INVALID; }
   if (d->precision != DG3D_FP64 && d->precision != DG3D_MIXED) { g_create_error = "bad precision"; return DG3D_E_INVALID; }
   if (d->precision == DG3D_MIXED && d->contraction != DG3D_CONTRACT_VALU) {
@@ -315,7 +330,7 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
       c->h_mat[i * 4 + 1] = 1000.0;
     }
     c->h_diff.assign((size_t)c->E, 0.0);
-    if (dmalloc(&c->d_mat, c->h_mat.size()) != hipSuccess || dmalloc(&c->d_diff, c->h_diff.size()) != hipSuccess ||
+    if (dmalloc(&c->d_mat, (size_t)c->E * dg3d::kMatStride) != hipSuccess || dmalloc(&c->d_diff, c->h_diff.size()) != hipSuccess ||
         dmalloc(&c->d_wq, c->n) != hipSuccess || dmalloc(&c->d_part, 1024 * 16) != hipSuccess) {
       g_create_error = "material allocation failed";
       free_ctx(c);

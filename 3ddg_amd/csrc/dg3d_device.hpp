@@ -18,14 +18,35 @@ enum Epilogue : int {
   EPI_RK33 = 2,   // out = (rhs dt beta + q beta) + old alpha
 };
 
+// Division of n < 2^31 by a runtime divisor d without a divide instruction:
+// q = (n * m) >> (31 + s) with s = ceil(log2 d), m = ceil(2^(31+s) / d) < 2^32.
+struct Div32 {
+  uint32_t m;
+  uint32_t s;
+};
+inline Div32 make_div32(uint32_t d) {
+  uint32_t s = 0;
+  while ((1ull << s) < d) ++s;
+  const uint64_t num = 1ull << (31 + s);
+  return Div32{(uint32_t)((num + d - 1) / d), s};
+}
+__host__ __device__ inline uint32_t div32(uint32_t n, Div32 d) {
+  return (uint32_t)(((uint64_t)n * d.m) >> (31 + d.s));
+}
+
+constexpr int kMatStride = 8;  // per-element material record (doubles)
+enum MatSlot { MAT_C = 0, MAT_RHO0 = 1, MAT_BA = 2, MAT_SIGMA = 3, MAT_IRHO0 = 4 };
+
 struct MeshArgs {
   int nE[3];
   int periodic[3];
-  int64_t E;          // interior elements
+  int64_t E;          // interior elements (< 2^31)
   int64_t fs;         // field stride in a register = E * n^3
   int64_t stride[3];  // element index stride per axis: e = (i*nE1 + j)*nE2 + k
+  Div32 div12;        // divide by nE1*nE2
+  Div32 div2;         // divide by nE2
   const double* inv_w[3];  // 1/width per element index along each axis
-  const double* mat;       // [E][4] = c0, rho0, B/A, sigma
+  const double* mat;       // [E][kMatStride] = c0, rho0, B/A, sigma, 1/rho0, -, -, -
 };
 
 struct ExtArgs {

@@ -71,27 +71,37 @@ __device__ __forceinline__ double source_value(const ExtArgs& x, double t) {
   return x.amp * sin(2.0 * M_PI * x.freq * t + x.phase) * exp(-a * a);
 }
 
-// Neighbour element along `axis` on `side`, or -1 for the exterior (halo) of the block.
-__device__ __forceinline__ int64_t neighbour(const MeshArgs& m, int64_t e, int axis, int side) {
-  const int64_t st = m.stride[axis];
-  const int64_t pos = (e / st) % m.nE[axis];
+// Element coordinates (i, j, k) of element e (< 2^31) by magic-number division.
+__device__ __forceinline__ void elem_coords(const MeshArgs& m, uint32_t e, int* c) {
+  const uint32_t i = div32(e, m.div12);
+  const uint32_t r = e - i * (uint32_t)(m.nE[1] * m.nE[2]);
+  const uint32_t j = div32(r, m.div2);
+  c[0] = (int)i;
+  c[1] = (int)j;
+  c[2] = (int)(r - j * (uint32_t)m.nE[2]);
+}
+
+// Neighbour element along `axis` on `side` given the element's coordinates, or -1 for the
+// exterior (halo) of the block.
+__device__ __forceinline__ int neighbour(const MeshArgs& m, int e, const int* ec, int axis, int side) {
+  const int st = (int)m.stride[axis];
+  const int pos = ec[axis];
   if (side == 0) {
     if (pos > 0) return e - st;
-    if (m.periodic[axis]) return e + (int64_t)(m.nE[axis] - 1) * st;
+    if (m.periodic[axis]) return e + (m.nE[axis] - 1) * st;
   } else {
     if (pos < m.nE[axis] - 1) return e + st;
-    if (m.periodic[axis]) return e - (int64_t)(m.nE[axis] - 1) * st;
+    if (m.periodic[axis]) return e - (m.nE[axis] - 1) * st;
   }
   return -1;
 }
 
-// Index of element e among the elements of a face normal to `axis` (the other two element
+// Index of an element among the elements of a face normal to `axis` (the other two element
 // coordinates in (i, j, k) order).
-__device__ __forceinline__ int64_t face_elem(const MeshArgs& m, int64_t e, int axis) {
-  const int64_t ei = e / m.stride[0], ej = (e / m.stride[1]) % m.nE[1], ek = e % m.nE[2];
-  if (axis == 0) return ej * m.nE[2] + ek;
-  if (axis == 1) return ei * m.nE[2] + ek;
-  return ei * m.nE[1] + ej;
+__device__ __forceinline__ int face_elem(const MeshArgs& m, const int* ec, int axis) {
+  if (axis == 0) return ec[1] * m.nE[2] + ec[2];
+  if (axis == 1) return ec[0] * m.nE[2] + ec[2];
+  return ec[0] * m.nE[1] + ec[1];
 }
 
 // Node index of the face node (b, c) at position `ia` along `axis`.
@@ -102,12 +112,22 @@ __device__ __forceinline__ int face_node(int axis, int ia, int b, int c) {
   return (b * NN + c) * NN + ia;
 }
 
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
+template <typename T>
+__device__ __forceinline__ void st_stream(T v, T* p) { __builtin_nontemporal_store(v, p); }
+
 template <int NN, typename CT, int EPI, typename DUT>
 __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
   using S = Shape<NN>;
   constexpr int NP = S::NP, NF2 = S::NF2, EPB = S::EPB, NT = S::NT;
-  __shared__ CT sF[EPB * 4 * NP];
-  __shared__ CT sFl[EPB * 6 * NF2 * 2];
+  constexpr int NITEMS = EPB * 6 * NF2;            // face work items of the workgroup
+  constexpr int R = (NITEMS + NT - 1) / NT;        // face items per lane
+  __shared__ CT sF[EPB * 4 * NP];                  // P/rho0, m_x, m_y, m_z per node
+  __shared__ CT sFl[EPB * 6 * NF2 * 2];            // flux pair per face node
+  __shared__ CT sNbM[EPB * 6 * 4];                 // neighbour c, rho0, B/A, 1/rho0 per face
+  __shared__ int sNbE[EPB * 6];                    // neighbour element or -1
+  __shared__ int sFe[EPB * 3];                     // face-element index per axis
   __shared__ CT sA[NN * NN];
   __shared__ CT sL0[NN];
   __shared__ CT sLN[NN];
@@ -115,7 +135,9 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
   const MeshArgs& m = L.mesh;
   const unsigned mask = L.mask;
   const int tid = threadIdx.x;
-  const int64_t e0 = remap_block(blockIdx.x, gridDim.x, L.xcd_remap) * EPB;
+  const int e0 = (int)(remap_block(blockIdx.x, gridDim.x, L.xcd_remap) * EPB);
+  const int E = (int)m.E;
+  const int64_t fs = m.fs;
 
   const CT* opA;
   const CT* opL0;
@@ -131,32 +153,89 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
     sLN[tid] = opLN[tid];
   }
 
+  // ---- phase 0: issue every global load of the stage before the first barrier ----------
   const int le = tid / NP;
   const int node = tid - le * NP;
-  const int64_t e = e0 + le;
-  const bool valid = (le < EPB) && (e < m.E);
-  const int64_t gi = e * NP + node;
+  const int e = e0 + le;
+  const bool valid = (le < EPB) && (e < E);
+  const int es = valid ? e : 0;
+  const int64_t gi = (int64_t)es * NP + node;
 
-  double q[4] = {0.0, 0.0, 0.0, 0.0};
-  double pre[4] = {0.0, 0.0, 0.0, 0.0};  // du (LSRK) or old (RK33), prefetched
-  CT c = CT(1), r0 = CT(1), ba = CT(0), sg = CT(0), ir0 = CT(1);
+  double q[4], pre[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int f = 0; f < 4; ++f) q[f] = L.q_in[f * fs + gi];
+  if (EPI == EPI_LSRK && L.a != 0.0) {
+    const DUT* dup = static_cast<const DUT*>(L.du);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) pre[f] = (double)ld_stream(dup + f * fs + gi);
+  }
+  if (EPI == EPI_RK33 && L.a != 0.0) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f) pre[f] = ld_stream(L.old + f * fs + gi);
+  }
+  const double* mp = m.mat + (int64_t)es * kMatStride;
+  const CT c = (CT)mp[MAT_C], r0 = (CT)mp[MAT_RHO0], ba = (CT)mp[MAT_BA], sg = (CT)mp[MAT_SIGMA];
+  const CT ir0 = (CT)mp[MAT_IRHO0];
+
+  // neighbour traces of this lane's face items (rho', u_axis), exterior items read the halo
+  double nrho[R], nu[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    nrho[r] = 0.0;
+    nu[r] = 0.0;
+    const int f = tid + r * NT;
+    const int fle = f / (6 * NF2);
+    const int rr = f - fle * 6 * NF2;
+    const int dir = rr / NF2;
+    const int fn = rr - dir * NF2;
+    const int axis = 2 - (dir >> 1);
+    const int side = dir & 1;
+    const int fe = e0 + fle;
+    const bool live = (f < NITEMS) && (fe < E) && (((mask >> (2 * axis)) & 3u) != 0u);
+    int ec[3];
+    elem_coords(m, live ? (uint32_t)fe : 0u, ec);
+    const int eo = live ? neighbour(m, fe, ec, axis, side) : -1;
+    const int b = fn / NN, cc = fn - b * NN;
+    const int on = face_node<NN>(axis, side ? 0 : NN - 1, b, cc);
+    const int esrc = (eo >= 0) ? eo : 0;
+    const double vr = L.q_in[(int64_t)esrc * NP + on];
+    const double vu = L.q_in[(1 + axis) * fs + (int64_t)esrc * NP + on];
+    if (eo >= 0) {
+      nrho[r] = vr;
+      nu[r] = vu;
+    } else if (live) {
+      const double* tr = L.ext.trace[dir];
+      if (tr) {
+        const int64_t ti = ((int64_t)face_elem(m, ec, axis) * NF2 + fn) * 3;
+        nrho[r] = tr[ti];
+        nu[r] = tr[ti + 1];
+      }
+    }
+  }
+  // per-element helper lanes: neighbour element and material of each face, face indices
+  if (tid < EPB * 6) {
+    const int hle = tid / 6, dir = tid - hle * 6;
+    const int he = e0 + hle;
+    const int axis = 2 - (dir >> 1);
+    int nbe = -1;
+    CT nm[4] = {CT(1), CT(1), CT(0), CT(1)};
+    if (he < E) {
+      int ec[3];
+      elem_coords(m, (uint32_t)he, ec);
+      nbe = neighbour(m, he, ec, axis, dir & 1);
+      // the halo element has the adjacent element's material (Block.cpp:167-248)
+      const double* nmp = m.mat + (int64_t)(nbe >= 0 ? nbe : he) * kMatStride;
+      nm[0] = (CT)nmp[MAT_C];
+      nm[1] = (CT)nmp[MAT_RHO0];
+      nm[2] = (CT)nmp[MAT_BA];
+      nm[3] = (CT)nmp[MAT_IRHO0];
+      if (dir < 3) sFe[hle * 3 + (2 - dir)] = face_elem(m, ec, 2 - dir);
+    }
+    sNbE[tid] = nbe;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sNbM[tid * 4 + k] = nm[k];
+  }
   if (valid) {
-#pragma unroll
-    for (int f = 0; f < 4; ++f) q[f] = L.q_in[f * m.fs + gi];
-    if (EPI == EPI_LSRK && L.a != 0.0) {
-#pragma unroll
-      for (int f = 0; f < 4; ++f) pre[f] = (double)static_cast<const DUT*>(L.du)[f * m.fs + gi];
-    }
-    if (EPI == EPI_RK33 && L.a != 0.0) {
-#pragma unroll
-      for (int f = 0; f < 4; ++f) pre[f] = L.old[f * m.fs + gi];
-    }
-    const double* mp = m.mat + e * 4;
-    c = (CT)mp[0];
-    r0 = (CT)mp[1];
-    ba = (CT)mp[2];
-    sg = (CT)mp[3];
-    ir0 = CT(1) / r0;
     const CT rho = (CT)q[0];
     CT* F = sF + le * 4 * NP;
     F[node] = momentum_f(rho, c, ba, ir0);
@@ -165,58 +244,39 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
   }
   __syncthreads();
 
-  // Face fluxes: one work item per (element, direction, face node).
-  for (int f = tid; f < EPB * 6 * NF2; f += NT) {
+  // ---- phase 1: upwind flux pair of every face node -------------------------------------
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int f = tid + r * NT;
+    if (f >= NITEMS) continue;
     const int fle = f / (6 * NF2);
-    const int r = f - fle * 6 * NF2;
-    const int dir = r / NF2;
-    const int fn = r - dir * NF2;
+    const int rr = f - fle * 6 * NF2;
+    const int dir = rr / NF2;
+    const int fn = rr - dir * NF2;
     const int axis = 2 - (dir >> 1);
     const int side = dir & 1;
-    if (((mask >> (2 * axis)) & 3u) == 0u) continue;
-    const int64_t fe = e0 + fle;
-    if (fe >= m.E) continue;
-    const int b = fn / NN, cc = fn - (fn / NN) * NN;
+    const int fe = e0 + fle;
+    if (fe >= E || ((mask >> (2 * axis)) & 3u) == 0u) continue;
+    const int b = fn / NN, cc = fn - b * NN;
     const int sn = face_node<NN>(axis, side ? NN - 1 : 0, b, cc);
-    const int on = face_node<NN>(axis, side ? 0 : NN - 1, b, cc);
-    const double* mps = m.mat + fe * 4;
-    const CT cs = (CT)mps[0], r0s = (CT)mps[1];
     const CT* F = sF + fle * 4 * NP;
     const CT mom_s = F[sn];
     const CT mass_s = F[(1 + axis) * NP + sn];
-    const int64_t eo = neighbour(m, fe, axis, side);
-    CT co, r0o, mom_o, mass_o;
-    if (eo >= 0) {
-      const double* mpo = m.mat + eo * 4;
-      co = (CT)mpo[0];
-      r0o = (CT)mpo[1];
-      const CT bao = (CT)mpo[2];
-      const CT rho_o = (CT)L.q_in[eo * NP + on];
-      const CT u_o = (CT)L.q_in[(1 + axis) * m.fs + eo * NP + on];
-      mom_o = momentum_f(rho_o, co, bao, CT(1) / r0o);
-      mass_o = mass_f(u_o, rho_o, r0o, bao);
-    } else {
-      // exterior: the halo element has the adjacent element's material (Block.cpp:167-248)
-      co = cs;
-      r0o = r0s;
-      const CT bao = (CT)mps[2];
-      double rho_o = 0.0, u_o = 0.0;
-      const int64_t fidx = face_elem(m, fe, axis);
-      const double* tr = L.ext.trace[dir];
-      if (tr) {
-        rho_o = tr[(fidx * NF2 + fn) * 3 + 0];
-        u_o = tr[(fidx * NF2 + fn) * 3 + 1];
-      }
-      if (dir == L.ext.src_face) {
-        const double t = L.ext.t + (L.ext.lead ? L.ext.lead[fidx * NF2 + fn] : 0.0);
-        const double s = source_value(L.ext, t);
-        const double cd = mps[0], rd = mps[1];
-        rho_o += s / (cd * cd);
-        u_o += (side ? -1.0 : 1.0) * s / (rd * cd);
-      }
-      mom_o = momentum_f((CT)rho_o, co, bao, CT(1) / r0o);
-      mass_o = mass_f((CT)u_o, (CT)rho_o, r0o, bao);
+    const double* mps = m.mat + (int64_t)fe * kMatStride;
+    const CT cs = (CT)mps[MAT_C], r0s = (CT)mps[MAT_RHO0];
+    const CT* nm = sNbM + (fle * 6 + dir) * 4;
+    const CT co = nm[0], r0o = nm[1], bao = nm[2], ir0o = nm[3];
+    double rho_o = nrho[r], u_o = nu[r];
+    if (sNbE[fle * 6 + dir] < 0 && dir == L.ext.src_face) {
+      const int fidx = sFe[fle * 3 + axis];
+      const double t = L.ext.t + (L.ext.lead ? L.ext.lead[(int64_t)fidx * NF2 + fn] : 0.0);
+      const double sv = source_value(L.ext, t);
+      const double cd = mps[MAT_C], rd = mps[MAT_RHO0];
+      rho_o += sv / (cd * cd);
+      u_o += (side ? -1.0 : 1.0) * sv / (rd * cd);
     }
+    const CT mom_o = momentum_f((CT)rho_o, co, bao, ir0o);
+    const CT mass_o = mass_f((CT)u_o, (CT)rho_o, r0o, bao);
     CT F0, F1;
     flux_pair<CT>(side, mom_s, mass_s, cs, r0s, mom_o, mass_o, co, r0o, F0, F1);
     CT* Fl = sFl + ((fle * 6 + dir) * NF2 + fn) * 2;
@@ -226,8 +286,10 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
   __syncthreads();
   if (!valid) return;
 
+  // ---- phase 2: line contractions, lifting, sponge, RK update ---------------------------
   const int idx[3] = {node / NF2, (node / NN) % NN, node % NN};
-  const int64_t pe[3] = {e / m.stride[0], (e / m.stride[1]) % m.nE[1], e % m.nE[2]};
+  int ecs[3];
+  elem_coords(m, (uint32_t)e, ecs);
   const CT* F = sF + le * 4 * NP;
   const CT* Fl = sFl + le * 6 * NF2 * 2;
   CT drho = CT(0);
@@ -249,7 +311,7 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
       dp += av * F[base + mm * ls];
       dm += av * Fa[base + mm * ls];
     }
-    const CT iw = (CT)m.inv_w[a][pe[a]];
+    const CT iw = (CT)m.inv_w[a][ecs[a]];
     const CT l0 = sL0[ix], ln = sLN[ix];
     const CT* flo = Fl + (dlo * NF2 + fn) * 2;
     const CT* fhi = Fl + ((dlo + 1) * NF2 + fn) * 2;
@@ -270,23 +332,23 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
     for (int a = 0; a < 3; ++a) touch[1 + a] = (mask & (kTermSponge | (1u << (2 * a)))) != 0u;
 #pragma unroll
     for (int f = 0; f < 4; ++f)
-      if (touch[f]) L.out[f * m.fs + gi] += (double)rhs[f];
+      if (touch[f]) L.out[f * fs + gi] += (double)rhs[f];
   } else if constexpr (EPI == EPI_LSRK) {
     DUT* dup = static_cast<DUT*>(L.du);
     if constexpr (std::is_same<DUT, double>::value) {
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
         const double dn = L.a * pre[f] + L.dt * (double)rhs[f];
-        dup[f * m.fs + gi] = dn;
-        L.q_out[f * m.fs + gi] = q[f] + L.b * dn;
+        st_stream(dn, dup + f * fs + gi);
+        st_stream(q[f] + L.b * dn, L.q_out + f * fs + gi);
       }
     } else {
       const float af = (float)L.a, dtf = (float)L.dt;
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
         const float dn = af * (float)pre[f] + dtf * (float)rhs[f];
-        dup[f * m.fs + gi] = dn;
-        L.q_out[f * m.fs + gi] = q[f] + L.b * (double)dn;
+        st_stream(dn, dup + f * fs + gi);
+        st_stream(q[f] + L.b * (double)dn, L.q_out + f * fs + gi);
       }
     }
   } else {
@@ -296,7 +358,7 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
     for (int f = 0; f < 4; ++f) {
       double v = (double)rhs[f] * sb + q[f] * L.b;
       if (L.a != 0.0) v += pre[f] * L.a;
-      L.out[f * m.fs + gi] = v;
+      L.out[f * fs + gi] = v;
     }
   }
 }
@@ -310,12 +372,12 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
   __shared__ double sFl[EPB * 2 * NF2];
   const MeshArgs& m = L.mesh;
   const int tid = threadIdx.x;
-  const int64_t e0 = (int64_t)blockIdx.x * EPB;
+  const int e0 = (int)blockIdx.x * EPB;
   const int le = tid / NP;
   const int node = tid - le * NP;
-  const int64_t e = e0 + le;
-  const bool valid = (le < EPB) && (e < m.E);
-  const int64_t gi = e * NP + node;
+  const int e = e0 + le;
+  const bool valid = (le < EPB) && (e < (int)m.E);
+  const int64_t gi = (int64_t)e * NP + node;
   const int axis = L.axis;
   const int fin = (L.which == 0) ? 0 : 4 + axis;  // rho' or Q_axis
   if (valid) sV[le * NP + node] = L.in[fin * m.fs + gi];
@@ -325,24 +387,26 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
     const int r = f - fle * 2 * NF2;
     const int side = r / NF2;
     const int fn = r - side * NF2;
-    const int64_t fe = e0 + fle;
-    if (fe >= m.E) continue;
+    const int fe = e0 + fle;
+    if (fe >= (int)m.E) continue;
     const int b = fn / NN, cc = fn % NN;
     const int sn = face_node<NN>(axis, side ? NN - 1 : 0, b, cc);
     const int on = face_node<NN>(axis, side ? 0 : NN - 1, b, cc);
     const double vs = sV[fle * NP + sn];
     double vo = 0.0;
-    const int64_t eo = neighbour(m, fe, axis, side);
+    int ec[3];
+    elem_coords(m, (uint32_t)fe, ec);
+    const int eo = neighbour(m, fe, ec, axis, side);
     const int dir = (axis == 0) ? 4 + side : (axis == 1 ? 2 + side : side);
     if (eo >= 0) {
-      vo = L.in[fin * m.fs + eo * NP + on];
+      vo = L.in[fin * m.fs + (int64_t)eo * NP + on];
     } else {
-      const int64_t fidx = face_elem(m, fe, axis);
+      const int64_t fidx = face_elem(m, ec, axis);
       const double* tr = L.ext.trace[dir];
       if (tr) vo = tr[(fidx * NF2 + fn) * 3 + (L.which == 0 ? 0 : 2)];
       if (L.which == 0 && dir == L.ext.src_face) {
         const double t = L.ext.t + (L.ext.lead ? L.ext.lead[fidx * NF2 + fn] : 0.0);
-        const double cd = m.mat[fe * 4 + 0];
+        const double cd = m.mat[(int64_t)fe * kMatStride + MAT_C];
         vo += source_value(L.ext, t) / (cd * cd);
       }
     }
@@ -359,8 +423,9 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
   double s = 0.0;
   for (int mm = 0; mm < NN; ++mm) s += L.opd[0][ix * NN + mm] * sV[le * NP + base + mm * ls];
   s += sFl[(le * 2 + 0) * NF2 + fn] * L.opd[1][ix] + sFl[(le * 2 + 1) * NF2 + fn] * L.opd[2][ix];
-  const int64_t pe = (axis == 0) ? e / m.stride[0] : (axis == 1 ? (e / m.stride[1]) % m.nE[1] : e % m.nE[2]);
-  const double x = s * m.inv_w[axis][pe];
+  int ec[3];
+  elem_coords(m, (uint32_t)e, ec);
+  const double x = s * m.inv_w[axis][ec[axis]];
   if (L.which == 0) {
     L.out[(4 + axis) * m.fs + gi] = -x;
   } else {
@@ -434,10 +499,11 @@ __global__ void __launch_bounds__(kRedThreads) k_energy(const double* reg, MeshA
     const int64_t e = i / np;
     const int nd = (int)(i - e * np);
     const int a = nd / (n * n), b = (nd / n) % n, c = nd % n;
-    const int64_t pe0 = e / m.stride[0], pe1 = (e / m.stride[1]) % m.nE[1], pe2 = e % m.nE[2];
-    const double jac = 0.125 / (m.inv_w[0][pe0] * m.inv_w[1][pe1] * m.inv_w[2][pe2]);
+    int ec[3];
+    elem_coords(m, (uint32_t)e, ec);
+    const double jac = 0.125 / (m.inv_w[0][ec[0]] * m.inv_w[1][ec[1]] * m.inv_w[2][ec[2]]);
     const double w = wq[a] * wq[b] * wq[c] * jac;
-    const double c0 = m.mat[e * 4 + 0], r0 = m.mat[e * 4 + 1];
+    const double c0 = m.mat[e * kMatStride + MAT_C], r0 = m.mat[e * kMatStride + MAT_RHO0];
     const double r = reg[i];
     const double u = reg[m.fs + i], v = reg[2 * m.fs + i], x = reg[3 * m.fs + i];
     en += w * 0.5 * (c0 * c0 * r * r / r0 + r0 * (u * u + v * v + x * x));
@@ -456,10 +522,11 @@ __global__ void __launch_bounds__(kRedThreads) k_cfl(MeshArgs m, double dxi_half
   auto mn = [](double x, double y) { return x < y ? x : y; };
   double v = 1e300;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m.E; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t pe0 = e / m.stride[0], pe1 = (e / m.stride[1]) % m.nE[1], pe2 = e % m.nE[2];
+    int ec[3];
+    elem_coords(m, (uint32_t)e, ec);
     // d-dimensional CFL: c dt sum_a 1/dx_a <= CFL with dx_a = w_a dxi_min / 2
-    const double iw = m.inv_w[0][pe0] + m.inv_w[1][pe1] + m.inv_w[2][pe2];
-    v = mn(v, dxi_half / (iw * m.mat[e * 4 + 0]));
+    const double iw = m.inv_w[0][ec[0]] + m.inv_w[1][ec[1]] + m.inv_w[2][ec[2]];
+    v = mn(v, dxi_half / (iw * m.mat[e * kMatStride + MAT_C]));
   }
   v = block_reduce(v, mn, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = v;
@@ -471,7 +538,8 @@ __global__ void k_probe(const double* reg, MeshArgs m, int n, const int64_t* nod
   const int64_t g = nodes[i];
   const int64_t np = (int64_t)n * n * n;
   const int64_t e = g / np;
-  const double c = m.mat[e * 4 + 0], r0 = m.mat[e * 4 + 1], ba = m.mat[e * 4 + 2];
+  const double c = m.mat[e * kMatStride + MAT_C], r0 = m.mat[e * kMatStride + MAT_RHO0];
+  const double ba = m.mat[e * kMatStride + MAT_BA];
   const double r = reg[g];
   out[i] = c * c * (r + ba * 0.5 / r0 * r * r);
 }

@@ -1,0 +1,231 @@
+"""Benchmark of the fused per-stage DG kernel on one MI355X.
+
+Workload (the configuration the metric is quoted on): one GPU's block of 64^3 hexahedra, N=4 (125
+LGL nodes each, 32.8M nodes x 4 FP64 fields), water with B/A=5, a 1 MHz 1e6 Pa tone burst entering
+through z=0, 8-element sponge, LSRK(5,4) at CFL 0.3; the FP64 drop-in path. A "step" is one
+LSRK54 step (5 fused stage kernels) over the whole block with state resident in HBM.
+
+Printed: one JSON line. Besides the headline value it carries the dominant kernel's roofline
+(algorithmic bytes per launch / HIP-event launch time), the CPU oracle timed on the host on a
+bounded sample, and the mixed-precision comparisons (same mesh at N=4 and at N=7: speed-up over
+FP64, rel L2 / peak pressure / second harmonic vs FP64 at t_end).
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per node-stage of the fused LSRK54 stage, averaged over the 5 stages
+# (stage 1 reads q, writes du and q: 96 B; stages 2-5 also read du: 128 B), FP64 / mixed
+BYTES_FP64 = (96.0 + 4 * 128.0) / 5.0   # 121.6
+BYTES_MIXED = (80.0 + 4 * 96.0) / 5.0   # 92.8
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--elements", type=int, default=64)
+    ap.add_argument("--no-extra", action="store_true", help="skip the mixed / N=7 comparison runs")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
+    return ap.parse_args()
+
+
+def calibrate_copy():
+    """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel give the scale factor
+    between the counters and HBM bytes (1 GiB read + 1 GiB written per dispatch)."""
+    import torch
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / 20
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copy_ms": el * 1e3,
+                      "copy_GBs": 2 * n * 8 / el / 1e9}))
+
+
+def timed_run(pk, p, precision, steps, warmup, probe=None):
+    ctx = pk.problem.make_context(p, precision=precision)
+    dt = ctx.cfl_dt(p.cfl)
+    if probe is not None:
+        ctx.set_probes([probe], steps + warmup)
+    t = 0.0
+    if warmup:
+        t = ctx.step(pk.dg3d.LSRK54, dt, warmup, t)
+    ctx.sync()
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t = ctx.step(pk.dg3d.LSRK54, dt, steps, t)
+    ctx.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    stage_ms, launches = ctx.last_step_timing()
+    return ctx, dict(elapsed=el, stage_ms=stage_ms, launches=launches, dt=dt, t_end=t)
+
+
+def probe_node(p, depth_frac=0.05):
+    """Global node index on the axis (x, y at the centre) near the source face."""
+    n = p.n
+    nE = p.num_elements
+    i, j = nE[0] // 2, nE[1] // 2
+    zc = np.concatenate([[0.0], np.cumsum(p.widths[2])])
+    k = int(np.searchsorted(zc, depth_frac * zc[-1])) - 1
+    k = max(0, min(nE[2] - 1, k))
+    e = (i * nE[1] + j) * nE[2] + k
+    return e * n ** 3 + (0 * n + 0) * n + 0
+
+
+def second_harmonic(trace, dt, f0):
+    """|DFT| of P(t) at 2 f0 over the recorded window (rectangular window), normalised."""
+    t = np.arange(len(trace)) * dt
+    return float(np.abs(np.sum(trace * np.exp(-2j * np.pi * 2 * f0 * t))) * 2.0 / max(len(trace), 1))
+
+
+def compare_precisions(pk, p, steps, warmup):
+    """FP64 and mixed runs of the same problem: speed and accuracy of mixed at t_end."""
+    pr = probe_node(p)
+    ctx64, r64 = timed_run(pk, p, pk.dg3d.FP64, steps, warmup, probe=pr)
+    s64 = ctx64.get_state(0)
+    tr64 = ctx64.get_probes()[:, 0]
+    ctx64.close()
+    ctx32, r32 = timed_run(pk, p, pk.dg3d.MIXED, steps, warmup, probe=pr)
+    s32 = ctx32.get_state(0)
+    tr32 = ctx32.get_probes()[:, 0]
+    ctx32.close()
+    a = s32.reshape(4, -1)
+    b = s64.reshape(4, -1)
+    rel = [float(np.linalg.norm(a[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-300)) for f in range(4)]
+    c2 = (p.c0 ** 2)[:, None] * np.ones((1, p.n ** 3))
+    ba = (p.b_over_a / (2 * p.rho0))[:, None]
+    P64 = (c2 * (b[0].reshape(p.E, -1) + ba * b[0].reshape(p.E, -1) ** 2))
+    P32 = (c2 * (a[0].reshape(p.E, -1) + ba * a[0].reshape(p.E, -1) ** 2))
+    pk64, pk32 = float(np.abs(P64).max()), float(np.abs(P32).max())
+    f0 = p.source.frequency
+    h64 = second_harmonic(tr64, r64["dt"], f0)
+    h32 = second_harmonic(tr32, r32["dt"], f0)
+    rate64 = p.dof * 5 * steps / r64["elapsed"]
+    rate32 = p.dof * 5 * steps / r32["elapsed"]
+    return dict(
+        workload=p.name, elements=list(p.num_elements), order=p.order, dof=p.dof, steps=steps,
+        fp64_dof_stage_per_s=rate64, mixed_dof_stage_per_s=rate32, mixed_speedup=rate32 / rate64,
+        fp64_stage_ms=r64["stage_ms"], mixed_stage_ms=r32["stage_ms"],
+        fp64_hbm_frac=p.dof * BYTES_FP64 / (r64["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        mixed_hbm_frac=p.dof * BYTES_MIXED / (r32["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        mixed_rel_l2_max=max(rel), mixed_rel_l2=rel,
+        peak_p_fp64=pk64, peak_p_rel_err=abs(pk32 - pk64) / max(pk64, 1e-300),
+        second_harmonic_fp64=h64, second_harmonic_rel_err=abs(h32 - h64) / max(h64, 1e-300),
+    )
+
+
+def cpu_baseline(p, steps):
+    """The CPU oracle (per-line dense solves, OpenMP over elements) on `steps` LSRK54 steps of the
+    same workload, on this host's cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Oracle, load_problem_into
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = Oracle(p, literal=False, num_registers=3, threads=threads)
+    load_problem_into(o, p)
+    dt = importlib.import_module("3ddg_amd").problem.cfl_dt_host(p)
+    o.set_time(0.0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.lsrk54(dt, 0, 1, 2)
+    el = time.perf_counter() - t0
+    del o
+    return dict(value=p.dof * 5 * steps / el, unit="DOF-stage/s", cores=threads, kind="port",
+                sample=f"{steps} LSRK54 step(s) (5 stages) of the full {p.name} block "
+                       f"({p.num_elements[0]}^3 elements, N={p.order}, {p.dof} nodes), FP64 oracle "
+                       f"with per-line dense solves, {el:.1f} s")
+
+
+def traffic_from_profiles(name):
+    path = os.path.join(ROOT, "profiles", f"{name}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("traffic_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    pk = importlib.import_module("3ddg_amd")
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    p = pk.problem.config_c2(num_elements=args.elements, steps=args.steps)
+    ctx, r = timed_run(pk, p, pk.dg3d.FP64, args.steps, args.warmup)
+    maxabs = ctx.reduce(pk.dg3d.REDUCE_MAXABS, 0)
+    ctx.close()
+    if not np.all(np.isfinite(maxabs[:4])):
+        print("non-finite state after the run", file=sys.stderr)
+        sys.exit(3)
+    value = p.dof * 5 * args.steps / r["elapsed"]
+    achieved = p.dof * BYTES_FP64 / (r["stage_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": "DOF-stage/s (FP64 drop-in fused LSRK54 stage; % HBM peak)",
+        "value": value,
+        "unit": "DOF-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["elapsed"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded tone burst, zero initial state)",
+        "config": {
+            "workload": f"{p.num_elements[0]}^3 hex elements, N={p.order} LGL, 4 FP64 fields, water B/A=5, "
+                        f"z=0 1 MHz 1e6 Pa tone burst, 8-element sponge, LSRK54 CFL {p.cfl}",
+            "elements": p.E, "order": p.order, "dof": p.dof, "scheme": "LSRK54", "dt": r["dt"],
+            "t_end": r["t_end"], "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("stage_fp64_c2"),
+            "kernel": "k_rhs<5,double,EPI_LSRK> (fused stage)", "avg_launch_ms": r["stage_ms"],
+            "bytes_per_node_stage": BYTES_FP64,
+        },
+    }
+    if not args.no_extra:
+        out["mixed_c2"] = compare_precisions(pk, pk.problem.config_c2(num_elements=args.elements, steps=args.steps),
+                                             args.steps, args.warmup)
+        out["mixed_c3"] = compare_precisions(pk, pk.problem.config_c3(num_elements=args.elements, steps=args.steps),
+                                             args.steps, args.warmup)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(pk.problem.config_c2(num_elements=args.elements), args.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
