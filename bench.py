@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--elements", type=int, default=64)
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed / N=7 comparison runs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     return ap.parse_args()
