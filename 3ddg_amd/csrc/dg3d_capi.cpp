@@ -62,6 +62,7 @@ struct dg3d_ctx {
   double* d_wq = nullptr;
   dg3d_source src{};
   double* d_lead = nullptr;
+  double* d_src_trace = nullptr;  // exterior trace of the source face at the current stage time
   int64_t* d_probe_nodes = nullptr;
   double* d_probe_buf = nullptr;
   int probe_count = 0, probe_cap = 0, probe_steps = 0;
@@ -116,6 +117,19 @@ struct dg3d_ctx {
     return x;
   }
   bool reg_ok(int r) const { return r >= 0 && r < (int)regs.size(); }
+  // Exterior arguments of the RHS kernel: with a source, a small kernel first writes the source
+  // face's trace (halo + source at time t) and the RHS kernel reads it as that face's halo.
+  hipError_t ext_rhs(int reg, double t, dg3d::ExtArgs* out) const {
+    dg3d::ExtArgs x = ext_for(reg, t);
+    if (src.face >= 0) {
+      hipError_t e = dg3d::launch_source_trace(x, mesh(), n, x.trace[src.face], d_src_trace, stream);
+      if (e != hipSuccess) return e;
+      x.trace[src.face] = d_src_trace;
+      x.src_face = -1;
+    }
+    *out = x;
+    return hipSuccess;
+  }
 };
 
 namespace {
@@ -144,6 +158,7 @@ void free_ctx(dg3d_ctx* c) {
   hipFree(c->d_diff);
   hipFree(c->d_wq);
   hipFree(c->d_lead);
+  hipFree(c->d_src_trace);
   hipFree(c->d_probe_nodes);
   hipFree(c->d_probe_buf);
   hipFree(c->d_part);
@@ -214,20 +229,23 @@ int run_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t, double** 
     L.du = c->work_du;
     L.a = kLsA[stage];
     L.b = kLsB[stage];
-    L.ext = c->ext_for(first ? 0 : -1, t + kLsC[stage] * dt);
+    hipError_t e = c->ext_rhs(first ? 0 : -1, t + kLsC[stage] * dt, &L.ext);
+    if (e != hipSuccess) return c->hip_fail(e, "source trace");
   } else {
     L.epilogue = dg3d::EPI_RK33;
     double* u0 = c->regs[0].d;
+    hipError_t e;
     if (stage == 0) {
       L.q_in = u0; L.out = c->work_k1; L.old = nullptr; L.a = 0.0; L.b = 1.0;
-      L.ext = c->ext_for(first ? 0 : -1, t);
+      e = c->ext_rhs(first ? 0 : -1, t, &L.ext);
     } else if (stage == 1) {
       L.q_in = c->work_k1; L.out = c->work_k2; L.old = u0; L.a = 0.75; L.b = 0.25;
-      L.ext = c->ext_for(-1, t + dt);
+      e = c->ext_rhs(-1, t + dt, &L.ext);
     } else {
       L.q_in = c->work_k2; L.out = u0; L.old = u0; L.a = 1.0 / 3.0; L.b = 2.0 / 3.0;
-      L.ext = c->ext_for(-1, t + 0.5 * dt);
+      e = c->ext_rhs(-1, t + 0.5 * dt, &L.ext);
     }
+    if (e != hipSuccess) return c->hip_fail(e, "source trace");
   }
   hipError_t e = dg3d::launch_rhs(L, c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "stage kernel launch");
@@ -477,8 +495,8 @@ int dg3d_operate(dg3d_ctx* c, int op, int axis, int mode, int in_reg, int out_re
     L.mask = (op == DG3D_OP_SPONGE) ? dg3d::kTermSponge : (1u << (2 * axis + mode));
     L.q_in = c->regs[in_reg].d;
     L.out = c->regs[out_reg].d;
-    L.ext = c->ext_for(in_reg, 0.0);
-    hipError_t e = dg3d::launch_rhs(L, c->stream);
+    hipError_t e = c->ext_rhs(in_reg, 0.0, &L.ext);
+    if (e == hipSuccess) e = dg3d::launch_rhs(L, c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "operate");
     return DG3D_OK;
   }
@@ -511,8 +529,11 @@ int dg3d_halo(dg3d_ctx* c, int reg) {
 
 int dg3d_set_source(dg3d_ctx* c, const dg3d_source* s) {
   if (!c) return DG3D_E_INVALID;
+  hipStreamSynchronize(c->stream);
   hipFree(c->d_lead);
+  hipFree(c->d_src_trace);
   c->d_lead = nullptr;
+  c->d_src_trace = nullptr;
   if (!s || s->face < 0) {
     c->src = dg3d_source{};
     c->src.face = -1;
@@ -523,6 +544,8 @@ int dg3d_set_source(dg3d_ctx* c, const dg3d_source* s) {
   if (!(s->tau > 0.0)) return c->fail(DG3D_E_INVALID, "source tau must be positive");
   c->src = *s;
   c->src.lead = nullptr;
+  if (dmalloc(&c->d_src_trace, (size_t)c->face_elems(s->face) * c->n * c->n * 3) != hipSuccess)
+    return c->fail(DG3D_E_NOMEM, "source allocation failed");
   if (s->lead) {
     const size_t cnt = (size_t)c->face_elems(s->face) * c->n * c->n;
     if (dmalloc(&c->d_lead, cnt) != hipSuccess) return c->fail(DG3D_E_NOMEM, "source allocation failed");

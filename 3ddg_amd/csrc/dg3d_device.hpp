@@ -84,6 +84,11 @@ struct RhsLaunch {
 
 hipError_t launch_rhs(const RhsLaunch& L, hipStream_t s);
 
+// Exterior trace of the source face at stage time x.t (halo trace + boundary source) into `out`
+// ([faceElem][n*n][3]); the RHS kernel then reads it as that face's halo.
+hipError_t launch_source_trace(const ExtArgs& x, const MeshArgs& m, int n, const double* halo, double* out,
+                               hipStream_t s);
+
 // LDG operators (Element.cpp:623-758) for one axis, FP64.
 struct LdgLaunch {
   int n;

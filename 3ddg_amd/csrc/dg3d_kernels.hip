@@ -50,22 +50,6 @@ __device__ __forceinline__ CT mass_f(CT u, CT rho, CT r0, CT ba) {
   return (ba == CT(0)) ? u * r0 : u * (rho + r0);
 }
 
-// Upwind acoustic flux pair of one line end (Element.cpp:540-600) with z = rho0 c and the
-// reference's coefficients simplified by z/rho0 = c (division-free).
-template <typename CT>
-__device__ __forceinline__ void flux_pair(int side, CT mom_s, CT mass_s, CT c1, CT r1, CT mom_o,
-                                          CT mass_o, CT c2, CT r2, CT& F0, CT& F1) {
-  const CT z1 = r1 * c1, z2 = r2 * c2;
-  const CT iz = CT(1) / (z1 + z2);
-  if (side == 0) {
-    F0 = -iz * (z1 * r2 * mom_o + z2 * r1 * mom_s + z1 * c2 * mass_o - z2 * c1 * mass_s);
-    F1 = -iz * (c2 * mass_o + c1 * mass_s + r2 * mom_o - r1 * mom_s);
-  } else {
-    F0 = iz * (z2 * r1 * mom_s + z1 * r2 * mom_o + z2 * c1 * mass_s - z1 * c2 * mass_o);
-    F1 = iz * (c1 * mass_s + c2 * mass_o + r1 * mom_s - r2 * mom_o);
-  }
-}
-
 __device__ __forceinline__ double source_value(const ExtArgs& x, double t) {
   const double a = (t - x.t0) / x.tau;
   return x.amp * sin(2.0 * M_PI * x.freq * t + x.phase) * exp(-a * a);
@@ -117,6 +101,61 @@ __device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontempora
 template <typename T>
 __device__ __forceinline__ void st_stream(T v, T* p) { __builtin_nontemporal_store(v, p); }
 
+// Face work items of a workgroup, (element, direction, face node) in that order: the same for
+// every workgroup, so they are tabulated once. x = element | dir << 6 | face node << 9,
+// y = self node | neighbour node << 16.
+template <int NN>
+struct ItemTable {
+  static constexpr int NITEMS = Shape<NN>::EPB * 6 * NN * NN;
+  unsigned x[NITEMS];
+  unsigned y[NITEMS];
+  constexpr ItemTable() : x(), y() {
+    for (int f = 0; f < NITEMS; ++f) {
+      const int fle = f / (6 * NN * NN), r = f % (6 * NN * NN), dir = r / (NN * NN), fn = r % (NN * NN);
+      const int axis = 2 - dir / 2, side = dir % 2, b = fn / NN, c = fn % NN;
+      const int ia = side ? NN - 1 : 0, io = side ? 0 : NN - 1;
+      const int sn = axis == 0 ? (ia * NN + b) * NN + c : (axis == 1 ? (b * NN + ia) * NN + c : (b * NN + c) * NN + ia);
+      const int on = axis == 0 ? (io * NN + b) * NN + c : (axis == 1 ? (b * NN + io) * NN + c : (b * NN + c) * NN + io);
+      x[f] = (unsigned)fle | ((unsigned)dir << 6) | ((unsigned)fn << 9);
+      y[f] = (unsigned)sn | ((unsigned)on << 16);
+    }
+  }
+};
+template <int NN>
+__device__ constexpr ItemTable<NN> kItems{};
+
+// Per (element, face) flux coefficients. With mom = P/rho0 and mass = m_axis of self (s) and
+// neighbour (o), the upwind pair of Element.cpp:540-600 scaled by the lift factors of
+// Element.cpp:602 (1/rho0 for the momentum equation, rho0 for the mass equation) is
+//   F0 = K0 mom_o + K1 mom_s + K2 mass_o + K3 mass_s
+//   F1 = K4 mass_o + K5 mass_s + K6 mom_o + K7 mom_s
+// and the neighbour's kernel functions are mom_o = (rho + rho^2 K8) K9, mass_o = u (rho K11 + K10).
+constexpr int kNK = 12;
+template <typename CT>
+__device__ __forceinline__ void face_coeffs(int side, CT c1, CT r1, CT ir1, CT c2, CT r2, CT ba2, CT ir2, CT* K) {
+  const CT z1 = r1 * c1, z2 = r2 * c2;
+  const CT iz = CT(1) / (z1 + z2);
+  const CT sg = side ? iz : -iz;  // the previous face carries the leading minus sign
+  const CT s0 = ir1, s1 = r1;
+  K[0] = sg * z1 * r2 * s0;
+  K[1] = sg * z2 * r1 * s0;
+  K[2] = -iz * z1 * c2 * s0;
+  K[3] = iz * z2 * c1 * s0;
+  K[4] = sg * c2 * s1;
+  K[5] = sg * c1 * s1;
+  K[6] = -iz * r2 * s1;
+  K[7] = iz * r1 * s1;
+  K[8] = ba2 * CT(0.5) * ir2;
+  K[9] = c2 * c2 * ir2;
+  K[10] = r2;
+  K[11] = (ba2 == CT(0)) ? CT(0) : CT(1);
+}
+
+template <typename T>
+__device__ __forceinline__ T sel3(int a, T x0, T x1, T x2) {
+  return a == 0 ? x0 : (a == 1 ? x1 : x2);
+}
+
 template <int NN, typename CT, int EPI, typename DUT>
 __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
   using S = Shape<NN>;
@@ -124,10 +163,10 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
   constexpr int NITEMS = EPB * 6 * NF2;            // face work items of the workgroup
   constexpr int R = (NITEMS + NT - 1) / NT;        // face items per lane
   __shared__ CT sF[EPB * 4 * NP];                  // P/rho0, m_x, m_y, m_z per node
-  __shared__ CT sFl[EPB * 6 * NF2 * 2];            // flux pair per face node
-  __shared__ CT sNbM[EPB * 6 * 4];                 // neighbour c, rho0, B/A, 1/rho0 per face
-  __shared__ int sNbE[EPB * 6];                    // neighbour element or -1
-  __shared__ int sFe[EPB * 3];                     // face-element index per axis
+  __shared__ CT sFl[EPB * 6 * NF2 * 2];            // scaled flux pair per face node
+  __shared__ CT sK[EPB * 6 * kNK];                 // flux coefficients per face
+  __shared__ int sNb[EPB * 6];                     // neighbour element, or -1 - face element index
+  __shared__ CT sIw[EPB * 3];                      // 1/width per element and axis
   __shared__ CT sA[NN * NN];
   __shared__ CT sL0[NN];
   __shared__ CT sLN[NN];
@@ -139,28 +178,12 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
   const int E = (int)m.E;
   const int64_t fs = m.fs;
 
-  const CT* opA;
-  const CT* opL0;
-  const CT* opLN;
-  if constexpr (std::is_same<CT, double>::value) {
-    opA = L.opd[0]; opL0 = L.opd[1]; opLN = L.opd[2];
-  } else {
-    opA = L.opf[0]; opL0 = L.opf[1]; opLN = L.opf[2];
-  }
-  if (tid < NN * NN) sA[tid] = opA[tid];
-  if (tid < NN) {
-    sL0[tid] = opL0[tid];
-    sLN[tid] = opLN[tid];
-  }
-
-  // ---- phase 0: issue every global load of the stage before the first barrier ----------
+  // ---- phase 0: the node's own loads; helper lanes find each face's neighbour ----------
   const int le = tid / NP;
   const int node = tid - le * NP;
   const int e = e0 + le;
   const bool valid = (le < EPB) && (e < E);
-  const int es = valid ? e : 0;
-  const int64_t gi = (int64_t)es * NP + node;
-
+  const int64_t gi = (int64_t)(valid ? e : 0) * NP + node;
   double q[4], pre[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int f = 0; f < 4; ++f) q[f] = L.q_in[f * fs + gi];
@@ -173,67 +196,93 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
 #pragma unroll
     for (int f = 0; f < 4; ++f) pre[f] = ld_stream(L.old + f * fs + gi);
   }
-  const double* mp = m.mat + (int64_t)es * kMatStride;
+  const double* mp = m.mat + (int64_t)(valid ? e : 0) * kMatStride;
   const CT c = (CT)mp[MAT_C], r0 = (CT)mp[MAT_RHO0], ba = (CT)mp[MAT_BA], sg = (CT)mp[MAT_SIGMA];
   const CT ir0 = (CT)mp[MAT_IRHO0];
-
-  // neighbour traces of this lane's face items (rho', u_axis), exterior items read the halo
-  double nrho[R], nu[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    nrho[r] = 0.0;
-    nu[r] = 0.0;
-    const int f = tid + r * NT;
-    const int fle = f / (6 * NF2);
-    const int rr = f - fle * 6 * NF2;
-    const int dir = rr / NF2;
-    const int fn = rr - dir * NF2;
-    const int axis = 2 - (dir >> 1);
-    const int side = dir & 1;
-    const int fe = e0 + fle;
-    const bool live = (f < NITEMS) && (fe < E) && (((mask >> (2 * axis)) & 3u) != 0u);
-    int ec[3];
-    elem_coords(m, live ? (uint32_t)fe : 0u, ec);
-    const int eo = live ? neighbour(m, fe, ec, axis, side) : -1;
-    const int b = fn / NN, cc = fn - b * NN;
-    const int on = face_node<NN>(axis, side ? 0 : NN - 1, b, cc);
-    const int esrc = (eo >= 0) ? eo : 0;
-    const double vr = L.q_in[(int64_t)esrc * NP + on];
-    const double vu = L.q_in[(1 + axis) * fs + (int64_t)esrc * NP + on];
-    if (eo >= 0) {
-      nrho[r] = vr;
-      nu[r] = vu;
-    } else if (live) {
-      const double* tr = L.ext.trace[dir];
-      if (tr) {
-        const int64_t ti = ((int64_t)face_elem(m, ec, axis) * NF2 + fn) * 3;
-        nrho[r] = tr[ti];
-        nu[r] = tr[ti + 1];
-      }
-    }
-  }
-  // per-element helper lanes: neighbour element and material of each face, face indices
   if (tid < EPB * 6) {
     const int hle = tid / 6, dir = tid - hle * 6;
     const int he = e0 + hle;
-    const int axis = 2 - (dir >> 1);
-    int nbe = -1;
-    CT nm[4] = {CT(1), CT(1), CT(0), CT(1)};
+    int nb = -1;
     if (he < E) {
+      const int axis = 2 - (dir >> 1);
       int ec[3];
       elem_coords(m, (uint32_t)he, ec);
-      nbe = neighbour(m, he, ec, axis, dir & 1);
-      // the halo element has the adjacent element's material (Block.cpp:167-248)
-      const double* nmp = m.mat + (int64_t)(nbe >= 0 ? nbe : he) * kMatStride;
-      nm[0] = (CT)nmp[MAT_C];
-      nm[1] = (CT)nmp[MAT_RHO0];
-      nm[2] = (CT)nmp[MAT_BA];
-      nm[3] = (CT)nmp[MAT_IRHO0];
-      if (dir < 3) sFe[hle * 3 + (2 - dir)] = face_elem(m, ec, 2 - dir);
+      const int pos = sel3(axis, ec[0], ec[1], ec[2]);
+      const int ne = sel3(axis, m.nE[0], m.nE[1], m.nE[2]);
+      const int st = (int)sel3(axis, m.stride[0], m.stride[1], m.stride[2]);
+      const int per = sel3(axis, m.periodic[0], m.periodic[1], m.periodic[2]);
+      const int fe = sel3(axis, ec[1] * m.nE[2] + ec[2], ec[0] * m.nE[2] + ec[2], ec[0] * m.nE[1] + ec[1]);
+      if (dir & 1) nb = (pos < ne - 1) ? he + st : (per ? he - (ne - 1) * st : -1 - fe);
+      else nb = (pos > 0) ? he - st : (per ? he + (ne - 1) * st : -1 - fe);
+      if ((dir & 1) == 0) sIw[hle * 3 + axis] = (CT)sel3(axis, m.inv_w[0], m.inv_w[1], m.inv_w[2])[pos];
     }
-    sNbE[tid] = nbe;
+    sNb[tid] = nb;
+  }
+  if (tid < NN * NN) {
+    if constexpr (std::is_same<CT, double>::value) sA[tid] = L.opd[0][tid];
+    else sA[tid] = L.opf[0][tid];
+  }
+  if (tid < NN) {
+    if constexpr (std::is_same<CT, double>::value) {
+      sL0[tid] = L.opd[1][tid];
+      sLN[tid] = L.opd[2][tid];
+    } else {
+      sL0[tid] = L.opf[1][tid];
+      sLN[tid] = L.opf[2][tid];
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 1: neighbour-trace loads, flux coefficients, the node's kernel functions ----
+  double nrho[R], nu[R];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) sNbM[tid * 4 + k] = nm[k];
+  for (int r = 0; r < R; ++r) {
+    const int f = tid + r * NT;
+    nrho[r] = 0.0;
+    nu[r] = 0.0;
+    if (f < NITEMS) {
+      const unsigned ix = kItems<NN>.x[f], iy = kItems<NN>.y[f];
+      const int fle = ix & 63u, dir = (ix >> 6) & 7u, fn = ix >> 9;
+      const int on = iy >> 16;
+      const int axis = 2 - (dir >> 1);
+      if (e0 + fle < E && ((mask >> (2 * axis)) & 3u) != 0u) {
+        const int nb = sNb[fle * 6 + dir];
+        if (nb >= 0) {
+          const int64_t oi = (int64_t)nb * NP + on;
+          nrho[r] = L.q_in[oi];
+          nu[r] = L.q_in[(1 + axis) * fs + oi];
+        } else {
+          const double* tr = L.ext.trace[0];
+          if (dir == 1) tr = L.ext.trace[1];
+          if (dir == 2) tr = L.ext.trace[2];
+          if (dir == 3) tr = L.ext.trace[3];
+          if (dir == 4) tr = L.ext.trace[4];
+          if (dir == 5) tr = L.ext.trace[5];
+          if (tr) {
+            const int64_t ti = ((int64_t)(-1 - nb) * NF2 + fn) * 3;
+            nrho[r] = tr[ti];
+            nu[r] = tr[ti + 1];
+          }
+        }
+      }
+    }
+  }
+  if (tid < EPB * 6) {
+    const int hle = tid / 6, dir = tid - hle * 6;
+    const int he = e0 + hle;
+    CT K[kNK];
+#pragma unroll
+    for (int k = 0; k < kNK; ++k) K[k] = CT(0);
+    if (he < E) {
+      const int nb = sNb[tid];
+      const double* ms = m.mat + (int64_t)he * kMatStride;
+      // the halo element has the adjacent element's material (Block.cpp:167-248)
+      const double* mo = m.mat + (int64_t)(nb >= 0 ? nb : he) * kMatStride;
+      face_coeffs<CT>(dir & 1, (CT)ms[MAT_C], (CT)ms[MAT_RHO0], (CT)ms[MAT_IRHO0], (CT)mo[MAT_C],
+                      (CT)mo[MAT_RHO0], (CT)mo[MAT_BA], (CT)mo[MAT_IRHO0], K);
+    }
+#pragma unroll
+    for (int k = 0; k < kNK; ++k) sK[tid * kNK + k] = K[k];
   }
   if (valid) {
     const CT rho = (CT)q[0];
@@ -244,52 +293,33 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
   }
   __syncthreads();
 
-  // ---- phase 1: upwind flux pair of every face node -------------------------------------
+  // ---- phase 2: upwind flux pair of every face node -------------------------------------
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int f = tid + r * NT;
     if (f >= NITEMS) continue;
-    const int fle = f / (6 * NF2);
-    const int rr = f - fle * 6 * NF2;
-    const int dir = rr / NF2;
-    const int fn = rr - dir * NF2;
+    const unsigned ix = kItems<NN>.x[f], iy = kItems<NN>.y[f];
+    const int fle = ix & 63u, dir = (ix >> 6) & 7u, fn = ix >> 9;
+    const int sn = iy & 0xFFFFu;
     const int axis = 2 - (dir >> 1);
-    const int side = dir & 1;
-    const int fe = e0 + fle;
-    if (fe >= E || ((mask >> (2 * axis)) & 3u) == 0u) continue;
-    const int b = fn / NN, cc = fn - b * NN;
-    const int sn = face_node<NN>(axis, side ? NN - 1 : 0, b, cc);
+    if (e0 + fle >= E || ((mask >> (2 * axis)) & 3u) == 0u) continue;
     const CT* F = sF + fle * 4 * NP;
     const CT mom_s = F[sn];
     const CT mass_s = F[(1 + axis) * NP + sn];
-    const double* mps = m.mat + (int64_t)fe * kMatStride;
-    const CT cs = (CT)mps[MAT_C], r0s = (CT)mps[MAT_RHO0];
-    const CT* nm = sNbM + (fle * 6 + dir) * 4;
-    const CT co = nm[0], r0o = nm[1], bao = nm[2], ir0o = nm[3];
-    double rho_o = nrho[r], u_o = nu[r];
-    if (sNbE[fle * 6 + dir] < 0 && dir == L.ext.src_face) {
-      const int fidx = sFe[fle * 3 + axis];
-      const double t = L.ext.t + (L.ext.lead ? L.ext.lead[(int64_t)fidx * NF2 + fn] : 0.0);
-      const double sv = source_value(L.ext, t);
-      const double cd = mps[MAT_C], rd = mps[MAT_RHO0];
-      rho_o += sv / (cd * cd);
-      u_o += (side ? -1.0 : 1.0) * sv / (rd * cd);
-    }
-    const CT mom_o = momentum_f((CT)rho_o, co, bao, ir0o);
-    const CT mass_o = mass_f((CT)u_o, (CT)rho_o, r0o, bao);
-    CT F0, F1;
-    flux_pair<CT>(side, mom_s, mass_s, cs, r0s, mom_o, mass_o, co, r0o, F0, F1);
+    const CT* K = sK + (fle * 6 + dir) * kNK;
+    const CT ro = (CT)nrho[r], uo = (CT)nu[r];
+    const CT mom_o = (ro + ro * ro * K[8]) * K[9];
+    const CT mass_o = uo * (ro * K[11] + K[10]);
     CT* Fl = sFl + ((fle * 6 + dir) * NF2 + fn) * 2;
-    Fl[0] = F0;
-    Fl[1] = F1;
+    Fl[0] = K[0] * mom_o + K[1] * mom_s + K[2] * mass_o + K[3] * mass_s;
+    Fl[1] = K[4] * mass_o + K[5] * mass_s + K[6] * mom_o + K[7] * mom_s;
   }
   __syncthreads();
+
   if (!valid) return;
 
-  // ---- phase 2: line contractions, lifting, sponge, RK update ---------------------------
+  // ---- phase 3: line contractions, lifting, sponge, RK update ---------------------------
   const int idx[3] = {node / NF2, (node / NN) % NN, node % NN};
-  int ecs[3];
-  elem_coords(m, (uint32_t)e, ecs);
   const CT* F = sF + le * 4 * NP;
   const CT* Fl = sFl + le * 6 * NF2 * 2;
   CT drho = CT(0);
@@ -311,12 +341,12 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
       dp += av * F[base + mm * ls];
       dm += av * Fa[base + mm * ls];
     }
-    const CT iw = (CT)m.inv_w[a][ecs[a]];
+    const CT iw = sIw[le * 3 + a];
     const CT l0 = sL0[ix], ln = sLN[ix];
     const CT* flo = Fl + (dlo * NF2 + fn) * 2;
     const CT* fhi = Fl + ((dlo + 1) * NF2 + fn) * 2;
-    if (am & 1u) du[a] = iw * (dp - (flo[0] * l0 + fhi[0] * ln) * ir0);
-    if (am & 2u) drho += iw * (dm - r0 * (flo[1] * l0 + fhi[1] * ln));
+    if (am & 1u) du[a] = iw * (dp - (flo[0] * l0 + fhi[0] * ln));
+    if (am & 2u) drho += iw * (dm - (flo[1] * l0 + fhi[1] * ln));
   }
   if (mask & kTermSponge) {
 #pragma unroll
@@ -361,6 +391,33 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
       L.out[f * fs + gi] = v;
     }
   }
+}
+
+// Exterior trace of the source face at one stage time: the face's halo trace (or zero) plus the
+// boundary source, p = s(t + lead), rho' = s/c^2, u_axis = +-s/(rho0 c) entering the block.
+__global__ void k_source_trace(ExtArgs x, MeshArgs m, int n, const double* halo, double* out) {
+  const int64_t nf2 = (int64_t)n * n;
+  const int face = x.src_face;
+  const int axis = 2 - face / 2;
+  const int nb = (axis == 0) ? m.nE[1] : m.nE[0];
+  const int nc = (axis == 2) ? m.nE[1] : m.nE[2];
+  const int64_t count = (int64_t)nb * nc * nf2;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int64_t fidx = i / nf2;
+  const int eb = (int)(fidx / nc), ec = (int)(fidx % nc);
+  int e3[3];
+  e3[axis] = (face & 1) ? m.nE[axis] - 1 : 0;
+  e3[axis == 0 ? 1 : 0] = eb;
+  e3[axis == 2 ? 1 : 2] = ec;
+  const int64_t e = ((int64_t)e3[0] * m.nE[1] + e3[1]) * m.nE[2] + e3[2];
+  const double t = x.t + (x.lead ? x.lead[i] : 0.0);
+  const double sv = source_value(x, t);
+  const double cd = m.mat[e * kMatStride + MAT_C], rd = m.mat[e * kMatStride + MAT_RHO0];
+  const double h0 = halo ? halo[i * 3] : 0.0, h1 = halo ? halo[i * 3 + 1] : 0.0, h2 = halo ? halo[i * 3 + 2] : 0.0;
+  out[i * 3 + 0] = h0 + sv / (cd * cd);
+  out[i * 3 + 1] = h1 + ((face & 1) ? -1.0 : 1.0) * sv / (rd * cd);
+  out[i * 3 + 2] = h2;
 }
 
 // LDG operators for one axis (Element.cpp:623-758), FP64, one lane per node.
@@ -598,6 +655,17 @@ hipError_t launch_ldg(const LdgLaunch& L, hipStream_t s) {
     case 8: return launch_ldg_n<8>(L, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_source_trace(const ExtArgs& x, const MeshArgs& m, int n, const double* halo, double* out,
+                               hipStream_t s) {
+  if (x.src_face < 0) return hipSuccess;
+  const int axis = 2 - x.src_face / 2;
+  const int64_t nb = (axis == 0) ? m.nE[1] : m.nE[0];
+  const int64_t nc = (axis == 2) ? m.nE[1] : m.nE[2];
+  const int64_t count = nb * nc * n * n;
+  hipLaunchKernelGGL(k_source_trace, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, x, m, n, halo, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_axpby(double* out, double a, const double* in, double b, int64_t count, hipStream_t s) {
