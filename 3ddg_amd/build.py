@@ -15,8 +15,10 @@ CSRC = os.path.join(PKG, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-# packed-FP32 instructions are disabled for every HIP translation unit
-DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# packed-FP32 instructions are disabled for every HIP translation unit; multiply-add fusion
+# follows the source expressions only (-ffp-contract=on), so every kernel variant computes
+# bitwise-identical results
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=on"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", f"-I{ROOT}/include", f"-I{CSRC}"]
 
 

@@ -69,6 +69,8 @@ struct dg3d_ctx {
   double* d_part = nullptr;
   std::vector<double> h_part;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int tune_variant[2] = {-1, -1};  // per scheme: chosen kernel variant (-1 = not tuned yet)
+  int tune_xcd[2] = {1, 1};
   double last_ms = 0.0;
   int last_launches = 0;
   std::string err;
@@ -210,6 +212,7 @@ dg3d::RhsLaunch base_launch(const dg3d_ctx* c) {
   L.n = c->n;
   L.mesh = c->mesh();
   L.xcd_remap = c->xcd;
+  L.variant = 0;
   for (int i = 0; i < 3; ++i) {
     L.opd[i] = c->d_opd[i];
     L.opf[i] = c->d_opf[i];
@@ -219,6 +222,10 @@ dg3d::RhsLaunch base_launch(const dg3d_ctx* c) {
 
 int run_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t, double** cur, double** other, bool first) {
   dg3d::RhsLaunch L = base_launch(c);
+  if (c->tune_variant[scheme] >= 0) {
+    L.variant = c->tune_variant[scheme];
+    L.xcd_remap = c->tune_xcd[scheme];
+  }
   L.mixed = (c->precision == DG3D_MIXED) ? 1 : 0;
   L.mask = dg3d::kTermAll;
   L.dt = dt;
@@ -262,9 +269,83 @@ int reduce_partials(dg3d_ctx* c, int per_block, int blocks) {
   return DG3D_OK;
 }
 
+// Times every (kernel variant, XCD remap) pair on launches of a stage that reads and writes the
+// full register set (LSRK54 stage 2; RK33 stage 2 with register 0 as both inputs). They write
+// only the work registers, which the next step's first stage overwrites without reading, so
+// tuning leaves the solution unchanged. The fastest pair is kept.
+int autotune(dg3d_ctx* c, int scheme) {
+  int rc = ensure_work(c, scheme);
+  if (rc) return rc;
+  double best = 1e300;
+  int bv = 0, bx = c->xcd;
+  const int reps = 4;
+  if (scheme == DG3D_LSRK54) {  // give the du register defined contents before it is read
+    hipError_t e = hipMemsetAsync(c->work_du, 0, (c->precision == DG3D_MIXED ? 4 : 8) * 4 * (size_t)c->fs, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "autotune");
+  }
+  for (int v = 0; v < dg3d::kNumVariants; ++v) {
+    for (int x = 0; x <= 1; ++x) {
+      dg3d::RhsLaunch L = base_launch(c);
+      L.variant = v;
+      L.xcd_remap = x;
+      L.mixed = (c->precision == DG3D_MIXED) ? 1 : 0;
+      L.mask = dg3d::kTermAll;
+      L.dt = 1e-30;
+      L.q_in = c->regs[0].d;
+      if (scheme == DG3D_LSRK54) {
+        L.epilogue = dg3d::EPI_LSRK;
+        L.q_out = c->work_q;
+        L.du = c->work_du;
+        L.a = kLsA[1];
+        L.b = kLsB[1];
+      } else {
+        L.epilogue = dg3d::EPI_RK33;
+        L.old = c->regs[0].d;
+        L.out = c->work_k2;
+        L.a = 0.75;
+        L.b = 0.25;
+      }
+      L.ext = c->ext_for(-1, 0.0);
+      hipError_t e = dg3d::launch_rhs(L, c->stream);  // warm-up (and first-use code load)
+      if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+      for (int r = 0; r < reps && e == hipSuccess; ++r) e = dg3d::launch_rhs(L, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+      if (e != hipSuccess) return c->hip_fail(e, "autotune");
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, c->ev0, c->ev1);
+      if (ms < best) {
+        best = ms;
+        bv = v;
+        bx = x;
+      }
+    }
+  }
+  c->tune_variant[scheme] = bv;
+  c->tune_xcd[scheme] = bx;
+  return DG3D_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int dg3d_autotune(dg3d_ctx* c, int scheme, int* variant, int* xcd_remap) {
+  if (!c || (scheme != DG3D_LSRK54 && scheme != DG3D_RK33)) return c ? c->fail(DG3D_E_INVALID, "autotune: bad scheme") : DG3D_E_INVALID;
+  const int rc = autotune(c, scheme);
+  if (rc) return rc;
+  if (variant) *variant = c->tune_variant[scheme];
+  if (xcd_remap) *xcd_remap = c->tune_xcd[scheme];
+  return DG3D_OK;
+}
+
+int dg3d_set_variant(dg3d_ctx* c, int scheme, int variant, int xcd_remap) {
+  if (!c || (scheme != DG3D_LSRK54 && scheme != DG3D_RK33)) return c ? c->fail(DG3D_E_INVALID, "set_variant: bad scheme") : DG3D_E_INVALID;
+  if (variant < 0 || variant >= dg3d::kNumVariants) return c->fail(DG3D_E_INVALID, "set_variant: variant out of range");
+  c->tune_variant[scheme] = variant;
+  c->tune_xcd[scheme] = xcd_remap ? 1 : 0;
+  return DG3D_OK;
+}
 
 int dg3d_abi_version(void) { return DG3D_ABI_VERSION; }
 const char* dg3d_create_error(void) { return g_create_error.c_str(); }
@@ -589,6 +670,10 @@ int dg3d_step(dg3d_ctx* c, int scheme, double dt, int nsteps, double* t) {
   int rc = ensure_work(c, scheme);
   if (rc) return rc;
   if (nsteps == 0) return DG3D_OK;
+  if (c->tune_variant[scheme] < 0) {
+    rc = autotune(c, scheme);
+    if (rc) return rc;
+  }
   const int ns = (scheme == DG3D_LSRK54) ? 5 : 3;
   double* cur = c->regs[0].d;
   double* other = c->work_q;

@@ -70,6 +70,7 @@ struct RhsLaunch {
   int epilogue;     // Epilogue
   unsigned mask;    // term mask
   int xcd_remap;
+  int variant;      // kernel shape variant 0..kNumVariants-1 (autotuned)
   MeshArgs mesh;
   ExtArgs ext;
   const double* opd[3];  // A, L0, LN in FP64
@@ -83,6 +84,7 @@ struct RhsLaunch {
 };
 
 hipError_t launch_rhs(const RhsLaunch& L, hipStream_t s);
+constexpr int kNumVariants = 4;
 
 // Exterior trace of the source face at stage time x.t (halo trace + boundary source) into `out`
 // ([faceElem][n*n][3]); the RHS kernel then reads it as that face's halo.

@@ -20,14 +20,26 @@
 namespace dg3d {
 namespace {
 
-template <int NN>
-struct Shape {
+// Workgroup shape: EPB whole elements of NN^3 nodes, one lane per node, padded to whole waves.
+template <int NN, int EPB_>
+struct Geo {
   static constexpr int NP = NN * NN * NN;
   static constexpr int NF2 = NN * NN;
-  // elements per workgroup: whole elements, lanes padded to whole waves with < 10% idle
-  static constexpr int EPB = (NN == 2) ? 32 : (NN == 3) ? 9 : (NN == 4) ? 4 : (NN == 5) ? 2
-                           : (NN == 6) ? 2 : (NN == 7) ? 2 : 1;
+  static constexpr int EPB = EPB_;
   static constexpr int NT = ((EPB * NP + 63) / 64) * 64;
+};
+// Default shape per order (whole waves with < 10% idle lanes); used by the LDG kernel.
+template <int NN>
+struct Shape : Geo<NN, (NN == 2) ? 32 : (NN == 3) ? 9 : (NN == 4) ? 4 : (NN == 5) ? 2 : (NN == 6) ? 2 : (NN == 7) ? 2 : 1> {};
+
+// Tunable variants of the stage kernel (chosen at run time by the autotuner of the C ABI):
+// bit 0 selects the alternative elements-per-workgroup, bit 1 the occupancy target (minimum
+// waves per SIMD the register allocation must allow).
+template <int NN>
+struct Variants {
+  static constexpr int EPB0 = Shape<NN>::EPB;
+  static constexpr int EPB1 = (NN == 2) ? 16 : (NN == 3) ? 7 : (NN == 4) ? 8 : (NN == 5) ? 4 : (NN == 8) ? 2 : 1;
+  static constexpr int MINW1 = (NN <= 5) ? 8 : 6;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so that XCD x
@@ -104,9 +116,9 @@ __device__ __forceinline__ void st_stream(T v, T* p) { __builtin_nontemporal_sto
 // Face work items of a workgroup, (element, direction, face node) in that order: the same for
 // every workgroup, so they are tabulated once. x = element | dir << 6 | face node << 9,
 // y = self node | neighbour node << 16.
-template <int NN>
+template <int NN, int EPB>
 struct ItemTable {
-  static constexpr int NITEMS = Shape<NN>::EPB * 6 * NN * NN;
+  static constexpr int NITEMS = EPB * 6 * NN * NN;
   unsigned x[NITEMS];
   unsigned y[NITEMS];
   constexpr ItemTable() : x(), y() {
@@ -121,8 +133,8 @@ struct ItemTable {
     }
   }
 };
-template <int NN>
-__device__ constexpr ItemTable<NN> kItems{};
+template <int NN, int EPB>
+__device__ constexpr ItemTable<NN, EPB> kItems{};
 
 // Per (element, face) flux coefficients. With mom = P/rho0 and mass = m_axis of self (s) and
 // neighbour (o), the upwind pair of Element.cpp:540-600 scaled by the lift factors of
@@ -156,10 +168,10 @@ __device__ __forceinline__ T sel3(int a, T x0, T x1, T x2) {
   return a == 0 ? x0 : (a == 1 ? x1 : x2);
 }
 
-template <int NN, typename CT, int EPI, typename DUT>
-__global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
-  using S = Shape<NN>;
-  constexpr int NP = S::NP, NF2 = S::NF2, EPB = S::EPB, NT = S::NT;
+template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT>
+__global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunch L) {
+  using S = Geo<NN, EPB>;
+  constexpr int NP = S::NP, NF2 = S::NF2, NT = S::NT;
   constexpr int NITEMS = EPB * 6 * NF2;            // face work items of the workgroup
   constexpr int R = (NITEMS + NT - 1) / NT;        // face items per lane
   __shared__ CT sF[EPB * 4 * NP];                  // P/rho0, m_x, m_y, m_z per node
@@ -241,7 +253,7 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
     nrho[r] = 0.0;
     nu[r] = 0.0;
     if (f < NITEMS) {
-      const unsigned ix = kItems<NN>.x[f], iy = kItems<NN>.y[f];
+      const unsigned ix = kItems<NN, EPB>.x[f], iy = kItems<NN, EPB>.y[f];
       const int fle = ix & 63u, dir = (ix >> 6) & 7u, fn = ix >> 9;
       const int on = iy >> 16;
       const int axis = 2 - (dir >> 1);
@@ -298,7 +310,7 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_rhs(const RhsLaunch L) {
   for (int r = 0; r < R; ++r) {
     const int f = tid + r * NT;
     if (f >= NITEMS) continue;
-    const unsigned ix = kItems<NN>.x[f], iy = kItems<NN>.y[f];
+    const unsigned ix = kItems<NN, EPB>.x[f], iy = kItems<NN, EPB>.y[f];
     const int fle = ix & 63u, dir = (ix >> 6) & 7u, fn = ix >> 9;
     const int sn = iy & 0xFFFFu;
     const int axis = 2 - (dir >> 1);
@@ -601,23 +613,37 @@ __global__ void k_probe(const double* reg, MeshArgs m, int n, const int64_t* nod
   out[i] = c * c * (r + ba * 0.5 / r0 * r * r);
 }
 
-template <int NN>
-hipError_t launch_rhs_n(const RhsLaunch& L, hipStream_t s) {
-  using S = Shape<NN>;
-  const int64_t blocks = (L.mesh.E + S::EPB - 1) / S::EPB;
+template <int NN, int EPB, int MINW>
+hipError_t launch_rhs_v(const RhsLaunch& L, hipStream_t s) {
+  using S = Geo<NN, EPB>;
+  const int64_t blocks = (L.mesh.E + EPB - 1) / EPB;
   if (blocks <= 0) return hipSuccess;
   dim3 grid((unsigned)blocks), block(S::NT);
   if (L.epilogue == EPI_ACCUM) {
     if (L.mixed) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_rhs<NN, double, EPI_ACCUM, double>), grid, block, 0, s, L);
+    hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_ACCUM, double>), grid, block, 0, s, L);
   } else if (L.epilogue == EPI_LSRK) {
-    if (L.mixed) hipLaunchKernelGGL((k_rhs<NN, float, EPI_LSRK, float>), grid, block, 0, s, L);
-    else hipLaunchKernelGGL((k_rhs<NN, double, EPI_LSRK, double>), grid, block, 0, s, L);
+    if (L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float>), grid, block, 0, s, L);
+    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_LSRK, double>), grid, block, 0, s, L);
   } else {
-    if (L.mixed) hipLaunchKernelGGL((k_rhs<NN, float, EPI_RK33, double>), grid, block, 0, s, L);
-    else hipLaunchKernelGGL((k_rhs<NN, double, EPI_RK33, double>), grid, block, 0, s, L);
+    if (L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double>), grid, block, 0, s, L);
+    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_RK33, double>), grid, block, 0, s, L);
   }
   return hipGetLastError();
+}
+
+template <int NN>
+hipError_t launch_rhs_n(const RhsLaunch& L, hipStream_t s) {
+  using V = Variants<NN>;
+  // the accumulate epilogue (Block::Operate*) only uses the default shape
+  const int v = (L.epilogue == EPI_ACCUM) ? 0 : L.variant;
+  switch (v) {
+    case 0: return launch_rhs_v<NN, V::EPB0, 1>(L, s);
+    case 1: return launch_rhs_v<NN, V::EPB1, 1>(L, s);
+    case 2: return launch_rhs_v<NN, V::EPB0, V::MINW1>(L, s);
+    case 3: return launch_rhs_v<NN, V::EPB1, V::MINW1>(L, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template <int NN>

@@ -74,6 +74,8 @@ _SIGS = {
     "dg3d_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
     "dg3d_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "dg3d_cfl_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _dp]),
+    "dg3d_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip]),
+    "dg3d_set_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "dg3d_reduce": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "dg3d_set_probes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
                                         ctypes.c_int]),
@@ -274,6 +276,14 @@ class Context:
 
     def stage(self, scheme, stage, dt, t):
         self._check(self._L.dg3d_stage(self._h, scheme, stage, float(dt), float(t)), "stage")
+
+    def autotune(self, scheme):
+        v, x = ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.dg3d_autotune(self._h, scheme, ctypes.byref(v), ctypes.byref(x)), "autotune")
+        return v.value, x.value
+
+    def set_variant(self, scheme, variant, xcd_remap):
+        self._check(self._L.dg3d_set_variant(self._h, scheme, int(variant), int(xcd_remap)), "set_variant")
 
     def cfl_dt(self, cfl):
         v = ctypes.c_double()

@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed / N=7 comparison runs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="pin the stage-kernel variant 0..3 (default: autotune), e.g. for profiled runs")
+    ap.add_argument("--xcd", type=int, default=1, help="XCD-contiguous numbering when --variant is pinned")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -62,9 +65,13 @@ def calibrate_copy():
                       "copy_GBs": 2 * n * 8 / el / 1e9}))
 
 
-def timed_run(pk, p, precision, steps, warmup, probe=None):
+def timed_run(pk, p, precision, steps, warmup, probe=None, variant=-1, xcd=1):
     ctx = pk.problem.make_context(p, precision=precision)
     dt = ctx.cfl_dt(p.cfl)
+    if variant >= 0:
+        ctx.set_variant(pk.dg3d.LSRK54, variant, xcd)
+    else:
+        variant, xcd = ctx.autotune(pk.dg3d.LSRK54)
     if probe is not None:
         ctx.set_probes([probe], steps + warmup)
     t = 0.0
@@ -79,7 +86,7 @@ def timed_run(pk, p, precision, steps, warmup, probe=None):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     stage_ms, launches = ctx.last_step_timing()
-    return ctx, dict(elapsed=el, stage_ms=stage_ms, launches=launches, dt=dt, t_end=t)
+    return ctx, dict(elapsed=el, stage_ms=stage_ms, launches=launches, dt=dt, t_end=t, variant=variant, xcd=xcd)
 
 
 def probe_node(p, depth_frac=0.05):
@@ -128,6 +135,7 @@ def compare_precisions(pk, p, steps, warmup):
         workload=p.name, elements=list(p.num_elements), order=p.order, dof=p.dof, steps=steps,
         fp64_dof_stage_per_s=rate64, mixed_dof_stage_per_s=rate32, mixed_speedup=rate32 / rate64,
         fp64_stage_ms=r64["stage_ms"], mixed_stage_ms=r32["stage_ms"],
+        fp64_variant=[r64["variant"], r64["xcd"]], mixed_variant=[r32["variant"], r32["xcd"]],
         fp64_hbm_frac=p.dof * BYTES_FP64 / (r64["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         mixed_hbm_frac=p.dof * BYTES_MIXED / (r32["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         mixed_rel_l2_max=max(rel), mixed_rel_l2=rel,
@@ -183,7 +191,7 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     p = pk.problem.config_c2(num_elements=args.elements, steps=args.steps)
-    ctx, r = timed_run(pk, p, pk.dg3d.FP64, args.steps, args.warmup)
+    ctx, r = timed_run(pk, p, pk.dg3d.FP64, args.steps, args.warmup, variant=args.variant, xcd=args.xcd)
     maxabs = ctx.reduce(pk.dg3d.REDUCE_MAXABS, 0)
     ctx.close()
     if not np.all(np.isfinite(maxabs[:4])):
@@ -213,7 +221,8 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("stage_fp64_c2"),
-            "kernel": "k_rhs<5,double,EPI_LSRK> (fused stage)", "avg_launch_ms": r["stage_ms"],
+            "kernel": "k_rhs<NN=5, EPB, MINW, double, EPI_LSRK, double> (fused stage)",
+            "variant": r["variant"], "xcd_remap": r["xcd"], "avg_launch_ms": r["stage_ms"],
             "bytes_per_node_stage": BYTES_FP64,
         },
     }

@@ -161,6 +161,14 @@ int dg3d_step(dg3d_ctx* ctx, int scheme, double dt, int nsteps, double* t);
 /* One stage of a scheme on register 0 (stage = 0..stages-1), for stage-level tests. */
 int dg3d_stage(dg3d_ctx* ctx, int scheme, int stage, double dt, double t);
 
+/* Autotuner of the fused stage kernel (no counterpart in the reference): times every kernel
+ * shape variant (elements per workgroup x occupancy target) with and without XCD-contiguous
+ * workgroup numbering on side-effect-free stage-0 launches and keeps the fastest for `scheme`.
+ * dg3d_step runs it on its first call unless a variant was pinned with dg3d_set_variant.
+ * Results do not depend on the variant (the per-node arithmetic is the same). */
+int dg3d_autotune(dg3d_ctx* ctx, int scheme, int* variant, int* xcd_remap);
+int dg3d_set_variant(dg3d_ctx* ctx, int scheme, int variant, int xcd_remap);
+
 /* CFL time step in FP64 on the device: dt = cfl / max_e( c(e) * sum_a 2 / (w_a(e) * dxi_min) ),
  * dxi_min the smallest LGL node gap on [-1,1] (the d-dimensional node-spacing CFL). */
 int dg3d_cfl_dt(dg3d_ctx* ctx, double cfl, double* dt);

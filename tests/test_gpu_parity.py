@@ -209,3 +209,27 @@ def test_reductions_and_cfl():
     ctx.set_state(1, st)
     d = ctx.reduce(dg3d.REDUCE_DIFFL2, 0, 1)
     assert np.isnan(d[0])
+
+
+@pytest.mark.parametrize("precision", [dg3d.FP64, dg3d.MIXED])
+@pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
+def test_kernel_variants_bitwise_identical(precision, scheme):
+    """Every autotuner candidate (elements per workgroup, occupancy target, XCD numbering)
+    computes the same per-node arithmetic, so the states agree bit for bit."""
+    print(f"kernel_variants precision={precision} scheme={scheme}", flush=True)
+    p = small_c2(n_e=5, order=4, steps=4)
+    dt = problem.cfl_dt_host(p)
+    ref = None
+    for v in range(4):
+        for x in (0, 1):
+            ctx = problem.make_context(p, precision=precision)
+            ctx.set_variant(scheme, v, x)
+            ctx.step(scheme, dt, p.steps, 0.0)
+            st = ctx.get_state(0)
+            ctx.close()
+            if ref is None:
+                ref = st
+            assert np.array_equal(st, ref), (v, x)
+    ctx = problem.make_context(p, precision=precision)
+    v, x = ctx.autotune(scheme)
+    assert 0 <= v < 4 and x in (0, 1)
