@@ -213,6 +213,7 @@ dg3d::RhsLaunch base_launch(const dg3d_ctx* c) {
   L.mesh = c->mesh();
   L.xcd_remap = c->xcd;
   L.variant = 0;
+  L.contraction = (c->precision == DG3D_MIXED) ? c->contraction : 0;
   for (int i = 0; i < 3; ++i) {
     L.opd[i] = c->d_opd[i];
     L.opf[i] = c->d_opf[i];
@@ -369,9 +370,9 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
   }
   if (d->num_registers < 1 || d->num_registers > 64) { g_create_error = "num_registers must be 1..64"; return DG3D_E_INVALID; }
   if (d->precision != DG3D_FP64 && d->precision != DG3D_MIXED) { g_create_error = "bad precision"; return DG3D_E_INVALID; }
-  if (d->precision == DG3D_MIXED && d->contraction != DG3D_CONTRACT_VALU) {
-    g_create_error = "MFMA contractions are not built into this library";
-    return DG3D_E_UNSUPPORTED;
+  if (d->contraction < DG3D_CONTRACT_VALU || d->contraction > DG3D_CONTRACT_MFMA_BF16) {
+    g_create_error = "bad contraction";
+    return DG3D_E_INVALID;
   }
   dg3d_ctx* c = new (std::nothrow) dg3d_ctx();
   if (!c) return DG3D_E_NOMEM;

@@ -71,6 +71,7 @@ struct RhsLaunch {
   unsigned mask;    // term mask
   int xcd_remap;
   int variant;      // kernel shape variant 0..kNumVariants-1 (autotuned)
+  int contraction;  // mixed path: 0 VALU, 1 MFMA FP32, 2 MFMA split-BF16
   MeshArgs mesh;
   ExtArgs ext;
   const double* opd[3];  // A, L0, LN in FP64

@@ -163,12 +163,158 @@ __device__ __forceinline__ void face_coeffs(int side, CT c1, CT r1, CT ir1, CT c
   K[11] = (ba2 == CT(0)) ? CT(0) : CT(1);
 }
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+
+// float -> bf16 bits, round to nearest even (finite values), and back
+__device__ __forceinline__ unsigned short f2bf(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+// x = hi + mid + lo as three bf16 pieces; both residuals are exact in FP32 and the
+// representation error is below 2^-26 |x|.
+__device__ __forceinline__ void split3(float x, short& h, short& m, short& l) {
+  const unsigned short a = f2bf(x);
+  const float r1 = x - bf2f(a);
+  const unsigned short b = f2bf(r1);
+  const float r2 = r1 - bf2f(b);
+  h = (short)a;
+  m = (short)b;
+  l = (short)f2bf(r2);
+}
+
+// Line of a contraction column: a column is (element, axis, field, face node) with the field
+// P/rho0 (f=0) or m_axis (f=1). fo/oo: LDS offset of the line's first node in sF and in sOut,
+// ls: node stride along the line.
+template <int NN>
+__device__ __forceinline__ void column_line(int col, int& fo, int& oo, int& ls) {
+  constexpr int NP = NN * NN * NN, NF2 = NN * NN;
+  const int le = col / (6 * NF2);
+  const int rr = col - le * 6 * NF2;
+  const int af = rr / NF2;
+  const int fn = rr - af * NF2;
+  const int a = af >> 1, f = af & 1;
+  const int b = fn / NN, c = fn - b * NN;
+  const int base = (a == 0) ? b * NN + c : (a == 1 ? b * NF2 + c : b * NF2 + c * NN);
+  ls = (a == 0) ? NF2 : (a == 1 ? NN : 1);
+  fo = le * 4 * NP + (f == 0 ? 0 : (1 + a)) * NP + base;
+  oo = (le * 6 + af) * NP + base;
+}
+
+// The workgroup's derivative contractions as one GEMM on the matrix cores:
+// out[i][col] = sum_m A[i][m] line_col[m], one column per (element, axis, field, face node).
+// P = 16/NN copies of A sit block-diagonally in the 16x16 operand, so one tile covers 16*P
+// columns (16 per diagonal block). Each wave takes tiles round-robin.
+//   CONTRACT 1: v_mfma_f32_16x16x4f32, FP32 operands.
+//   CONTRACT 2: v_mfma_f32_16x16x16bf16_1k with A and the lines split into three bf16 pieces
+//               each; the six products of combined order <= 2 are accumulated in FP32,
+//               smallest first.
+// Lane layouts (16x16 tile, lane l, g = l/16, j = l%16):
+//   16x16x4f32:        A[j][k = g], B[k = g][j], D[4g + r][j]
+//   16x16x16bf16_1k:   A[j][k = 4g + t], B[k = 4g + t][j], D[4g + r][j]
+template <int NN, int EPB, int NT, int CONTRACT>
+__device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf, float* sOut, int tid) {
+  constexpr int P = 16 / NN;
+  constexpr int C = EPB * 6 * NN * NN;
+  constexpr int TC = 16 * P;
+  constexpr int T = (C + TC - 1) / TC;
+  constexpr int W = NT / 64;
+  static_assert(NT % 64 == 0, "whole waves");
+  const int lane = tid & 63, wave = tid >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  if constexpr (CONTRACT == 1) {
+    constexpr int KS = (P * NN + 3) / 4;
+    float aop[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int k = 4 * s + g;
+      const bool in = (j / NN == k / NN) && (j / NN < P) && (k < P * NN);
+      aop[s] = in ? sAf[(j % NN) * NN + (k % NN)] : 0.f;
+    }
+    for (int t = wave; t < T; t += W) {
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + g;
+        const int col = t * TC + (k / NN) * 16 + j;
+        float bv = 0.f;
+        if (k < P * NN && col < C) {
+          int fo, oo, ls;
+          column_line<NN>(col, fo, oo, ls);
+          bv = sF[fo + (k % NN) * ls];
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aop[s], bv, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ip = 4 * g + r;
+        const int col = t * TC + (ip / NN) * 16 + j;
+        if (ip / NN < P && col < C) {
+          int fo, oo, ls;
+          column_line<NN>(col, fo, oo, ls);
+          sOut[oo + (ip % NN) * ls] = acc[r];
+        }
+      }
+    }
+  } else {
+    bf16x4 ah, am, al;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = 4 * g + q;
+      const bool in = (j / NN == k / NN) && (j / NN < P) && (k < P * NN);
+      short h, m, l;
+      split3(in ? sAf[(j % NN) * NN + (k % NN)] : 0.f, h, m, l);
+      ah[q] = h;
+      am[q] = m;
+      al[q] = l;
+    }
+    for (int t = wave; t < T; t += W) {
+      bf16x4 bh, bm, bl;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = 4 * g + q;
+        const int col = t * TC + (k / NN) * 16 + j;
+        float bv = 0.f;
+        if (k < P * NN && col < C) {
+          int fo, oo, ls;
+          column_line<NN>(col, fo, oo, ls);
+          bv = sF[fo + (k % NN) * ls];
+        }
+        short h, m, l;
+        split3(bv, h, m, l);
+        bh[q] = h;
+        bm[q] = m;
+        bl[q] = l;
+      }
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bm, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bm, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ip = 4 * g + r;
+        const int col = t * TC + (ip / NN) * 16 + j;
+        if (ip / NN < P && col < C) {
+          int fo, oo, ls;
+          column_line<NN>(col, fo, oo, ls);
+          sOut[oo + (ip % NN) * ls] = acc[r];
+        }
+      }
+    }
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T sel3(int a, T x0, T x1, T x2) {
   return a == 0 ? x0 : (a == 1 ? x1 : x2);
 }
 
-template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT>
+template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT, int CONTRACT = 0>
 __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunch L) {
   using S = Geo<NN, EPB>;
   constexpr int NP = S::NP, NF2 = S::NF2, NT = S::NT;
@@ -179,6 +325,7 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
   __shared__ CT sK[EPB * 6 * kNK];                 // flux coefficients per face
   __shared__ int sNb[EPB * 6];                     // neighbour element, or -1 - face element index
   __shared__ CT sIw[EPB * 3];                      // 1/width per element and axis
+  __shared__ float sOut[CONTRACT ? EPB * 6 * NP : 1];  // matrix-core contraction results
   __shared__ CT sA[NN * NN];
   __shared__ CT sL0[NN];
   __shared__ CT sLN[NN];
@@ -327,6 +474,10 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
     Fl[1] = K[4] * mass_o + K[5] * mass_s + K[6] * mom_o + K[7] * mom_s;
   }
   __syncthreads();
+  if constexpr (CONTRACT != 0) {
+    contract_mfma<NN, EPB, NT, CONTRACT>(sF, sA, sOut, tid);
+    __syncthreads();
+  }
 
   if (!valid) return;
 
@@ -347,11 +498,16 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
     const int dlo = (a == 0) ? 4 : (a == 1 ? 2 : 0);
     const CT* Fa = F + (1 + a) * NP;
     CT dp = CT(0), dm = CT(0);
+    if constexpr (CONTRACT != 0) {
+      dp = sOut[(le * 6 + 2 * a) * NP + node];
+      dm = sOut[(le * 6 + 2 * a + 1) * NP + node];
+    } else {
 #pragma unroll
-    for (int mm = 0; mm < NN; ++mm) {
-      const CT av = sA[ix * NN + mm];
-      dp += av * F[base + mm * ls];
-      dm += av * Fa[base + mm * ls];
+      for (int mm = 0; mm < NN; ++mm) {
+        const CT av = sA[ix * NN + mm];
+        dp += av * F[base + mm * ls];
+        dm += av * Fa[base + mm * ls];
+      }
     }
     const CT iw = sIw[le * 3 + a];
     const CT l0 = sL0[ix], ln = sLN[ix];
@@ -623,11 +779,15 @@ hipError_t launch_rhs_v(const RhsLaunch& L, hipStream_t s) {
     if (L.mixed) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_ACCUM, double>), grid, block, 0, s, L);
   } else if (L.epilogue == EPI_LSRK) {
-    if (L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float>), grid, block, 0, s, L);
-    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_LSRK, double>), grid, block, 0, s, L);
+    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_LSRK, double>), grid, block, 0, s, L);
+    else if (L.contraction == 1) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float, 1>), grid, block, 0, s, L);
+    else if (L.contraction == 2) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float, 2>), grid, block, 0, s, L);
+    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float>), grid, block, 0, s, L);
   } else {
-    if (L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double>), grid, block, 0, s, L);
-    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_RK33, double>), grid, block, 0, s, L);
+    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_RK33, double>), grid, block, 0, s, L);
+    else if (L.contraction == 1) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double, 1>), grid, block, 0, s, L);
+    else if (L.contraction == 2) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double, 2>), grid, block, 0, s, L);
+    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double>), grid, block, 0, s, L);
   }
   return hipGetLastError();
 }

@@ -65,8 +65,8 @@ def calibrate_copy():
                       "copy_GBs": 2 * n * 8 / el / 1e9}))
 
 
-def timed_run(pk, p, precision, steps, warmup, probe=None, variant=-1, xcd=1):
-    ctx = pk.problem.make_context(p, precision=precision)
+def timed_run(pk, p, precision, steps, warmup, probe=None, variant=-1, xcd=1, contraction=0):
+    ctx = pk.problem.make_context(p, precision=precision, contraction=contraction)
     dt = ctx.cfl_dt(p.cfl)
     if variant >= 0:
         ctx.set_variant(pk.dg3d.LSRK54, variant, xcd)
@@ -131,6 +131,14 @@ def compare_precisions(pk, p, steps, warmup):
     h32 = second_harmonic(tr32, r32["dt"], f0)
     rate64 = p.dof * 5 * steps / r64["elapsed"]
     rate32 = p.dof * 5 * steps / r32["elapsed"]
+    matrix_core = {}
+    for name, con in (("mfma_f32", pk.dg3d.CONTRACT_MFMA_F32), ("mfma_bf16x3", pk.dg3d.CONTRACT_MFMA_BF16)):
+        ctxm, rm = timed_run(pk, p, pk.dg3d.MIXED, steps, warmup, contraction=con)
+        sm = ctxm.get_state(0).reshape(4, -1)
+        ctxm.close()
+        relm = [float(np.linalg.norm(sm[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-300)) for f in range(4)]
+        matrix_core[name] = dict(stage_ms=rm["stage_ms"], variant=[rm["variant"], rm["xcd"]],
+                                 speedup_vs_fp64=r64["stage_ms"] / rm["stage_ms"], rel_l2_max=max(relm))
     return dict(
         workload=p.name, elements=list(p.num_elements), order=p.order, dof=p.dof, steps=steps,
         fp64_dof_stage_per_s=rate64, mixed_dof_stage_per_s=rate32, mixed_speedup=rate32 / rate64,
@@ -141,6 +149,7 @@ def compare_precisions(pk, p, steps, warmup):
         mixed_rel_l2_max=max(rel), mixed_rel_l2=rel,
         peak_p_fp64=pk64, peak_p_rel_err=abs(pk32 - pk64) / max(pk64, 1e-300),
         second_harmonic_fp64=h64, second_harmonic_rel_err=abs(h32 - h64) / max(h64, 1e-300),
+        matrix_core_contraction=matrix_core,
     )
 
 

@@ -233,3 +233,53 @@ def test_kernel_variants_bitwise_identical(precision, scheme):
     ctx = problem.make_context(p, precision=precision)
     v, x = ctx.autotune(scheme)
     assert 0 <= v < 4 and x in (0, 1)
+
+
+@pytest.mark.parametrize("contraction", [dg3d.CONTRACT_MFMA_F32, dg3d.CONTRACT_MFMA_BF16])
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6, 7])
+def test_mixed_matrix_core_contraction(contraction, order):
+    """The mixed step with the derivative contractions on the matrix cores (FP32 operands, or
+    three-piece split-BF16) stays within the mixed tolerance of the FP64 step, and within FP32
+    rounding of the same step with VALU contractions."""
+    print(f"mixed_matrix_core contraction={contraction} order={order}", flush=True)
+    p = heterogeneous(small_c2(n_e=4, order=order, steps=60))
+    dt = problem.cfl_dt_host(p)
+    st = {}
+    for name, prec, con in (("fp64", dg3d.FP64, dg3d.CONTRACT_VALU), ("valu", dg3d.MIXED, dg3d.CONTRACT_VALU),
+                            ("mfma", dg3d.MIXED, contraction)):
+        ctx = problem.make_context(p, precision=prec, contraction=con)
+        ctx.step(dg3d.LSRK54, dt, p.steps, 0.0)
+        st[name] = ctx.get_state(0)
+        ctx.close()
+    assert np.linalg.norm(st["fp64"]) > 0
+    err64 = rel_l2(st["mfma"], st["fp64"])
+    errv = rel_l2(st["mfma"], st["valu"])
+    assert err64.max() < TOL_MIXED, err64
+    assert errv.max() < 5e-6, errv
+
+
+@pytest.mark.parametrize("contraction", [dg3d.CONTRACT_MFMA_F32, dg3d.CONTRACT_MFMA_BF16])
+@pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
+def test_matrix_core_variants(contraction, scheme):
+    """Each variant is deterministic (two runs agree bit for bit). Across variants the elements
+    per workgroup change which diagonal block of the 16x16 tile a line lands in, and with it the
+    order in which the matrix core sums the line's products, so variants agree to FP32 rounding."""
+    print(f"matrix_core_variants contraction={contraction} scheme={scheme}", flush=True)
+    p = small_c2(n_e=5, order=4, steps=4)
+    dt = problem.cfl_dt_host(p)
+    ref = None
+    for v in range(4):
+        for x in (0, 1):
+            runs = []
+            for _ in range(2 if x == 0 else 1):
+                ctx = problem.make_context(p, precision=dg3d.MIXED, contraction=contraction)
+                ctx.set_variant(scheme, v, x)
+                ctx.step(scheme, dt, p.steps, 0.0)
+                runs.append(ctx.get_state(0))
+                ctx.close()
+            if len(runs) == 2:
+                assert np.array_equal(runs[0], runs[1]), (v, x)
+            if ref is None:
+                ref = runs[0]
+            err = rel_l2(runs[0], ref)
+            assert err.max() < 1e-6, (v, x, err)
