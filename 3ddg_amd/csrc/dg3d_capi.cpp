@@ -37,7 +37,7 @@ struct Register {
 }  // namespace
 
 struct dg3d_ctx {
-  int N = 0, n = 0, np = 0;
+  int N = 0, n = 0, np = 0, es = 0;  // es: node slots per element in device memory
   int nE[3] = {0, 0, 0};
   int periodic[3] = {0, 0, 0};
   int64_t E = 0, fs = 0;
@@ -97,6 +97,7 @@ struct dg3d_ctx {
     }
     m.E = E;
     m.fs = fs;
+    m.es = es;
     m.stride[0] = (int64_t)nE[1] * nE[2];
     m.stride[1] = nE[2];
     m.stride[2] = 1;
@@ -364,8 +365,11 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
   }
   for (int a = 0; a < 3; ++a)
     if (d->num_elements[a] < 1) { g_create_error = "num_elements must be >= 1"; return DG3D_E_INVALID; }
-  if ((int64_t)d->num_elements[0] * d->num_elements[1] * d->num_elements[2] >= (1ll << 31)) {
-    g_create_error = "more than 2^31 elements in one block";
+  if (d->order >= 1 && d->order <= 7 &&
+      dg3d::field_slots((int64_t)d->num_elements[0] * d->num_elements[1] * d->num_elements[2], d->order + 1) >=
+          (1ll << 29)) {
+    // byte offsets within one field array are 32-bit in the kernels
+    g_create_error = "2^29 or more nodes in one block: split the domain into blocks (NUM_BLOCKS)";
     return DG3D_E_INVALID;
   }
   if (d->num_registers < 1 || d->num_registers > 64) { g_create_error = "num_registers must be 1..64"; return DG3D_E_INVALID; }
@@ -385,7 +389,8 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
       c->periodic[a] = d->periodic[a] ? 1 : 0;
     }
     c->E = (int64_t)c->nE[0] * c->nE[1] * c->nE[2];
-    c->fs = c->E * c->np;
+    c->es = dg3d::elem_stride(c->n);
+    c->fs = dg3d::field_slots(c->E, c->n);
     c->nfields = d->with_q ? 7 : 4;
     c->precision = d->precision;
     c->contraction = d->contraction;
@@ -500,7 +505,18 @@ int dg3d_set_material(dg3d_ctx* c, const double* c0, const double* rho0, const d
 int dg3d_set_state(dg3d_ctx* c, int reg, const double* host, int nfields) {
   if (!c || !c->reg_ok(reg) || !host) return c ? c->fail(DG3D_E_INVALID, "set_state: bad register or pointer") : DG3D_E_INVALID;
   if (nfields < 1 || nfields > c->nfields) return c->fail(DG3D_E_INVALID, "set_state: nfields exceeds the register");
-  hipError_t e = hipMemcpy(c->regs[reg].d, host, sizeof(double) * nfields * c->fs, hipMemcpyHostToDevice);
+  hipError_t e = hipSuccess;
+  if (c->es == c->np && c->fs == c->E * c->np) {
+    e = hipMemcpy(c->regs[reg].d, host, sizeof(double) * nfields * c->fs, hipMemcpyHostToDevice);
+  } else {
+    // host fields are [E][n^3]; the device reserves es slots per element (zeros in the padding)
+    std::vector<double> stage((size_t)nfields * c->fs, 0.0);
+    for (int f = 0; f < nfields; ++f)
+      for (int64_t el = 0; el < c->E; ++el)
+        std::memcpy(&stage[(size_t)f * c->fs + (size_t)el * c->es], host + ((size_t)f * c->E + el) * c->np,
+                    sizeof(double) * c->np);
+    e = hipMemcpy(c->regs[reg].d, stage.data(), sizeof(double) * stage.size(), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) return c->hip_fail(e, "set_state");
   return DG3D_OK;
 }
@@ -509,7 +525,18 @@ int dg3d_get_state(dg3d_ctx* c, int reg, double* host, int nfields) {
   if (!c || !c->reg_ok(reg) || !host) return c ? c->fail(DG3D_E_INVALID, "get_state: bad register or pointer") : DG3D_E_INVALID;
   if (nfields < 1 || nfields > c->nfields) return c->fail(DG3D_E_INVALID, "get_state: nfields exceeds the register");
   hipError_t e = hipStreamSynchronize(c->stream);
-  if (e == hipSuccess) e = hipMemcpy(host, c->regs[reg].d, sizeof(double) * nfields * c->fs, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) {
+    if (c->es == c->np && c->fs == c->E * c->np) {
+      e = hipMemcpy(host, c->regs[reg].d, sizeof(double) * nfields * c->fs, hipMemcpyDeviceToHost);
+    } else {
+      std::vector<double> stage((size_t)nfields * c->fs);
+      e = hipMemcpy(stage.data(), c->regs[reg].d, sizeof(double) * stage.size(), hipMemcpyDeviceToHost);
+      for (int f = 0; f < nfields && e == hipSuccess; ++f)
+        for (int64_t el = 0; el < c->E; ++el)
+          std::memcpy(host + ((size_t)f * c->E + el) * c->np, &stage[(size_t)f * c->fs + (size_t)el * c->es],
+                      sizeof(double) * c->np);
+    }
+  }
   if (e != hipSuccess) return c->hip_fail(e, "get_state");
   return DG3D_OK;
 }
@@ -805,12 +832,15 @@ int dg3d_set_probes(dg3d_ctx* c, int count, const int64_t* nodes, int capacity_s
   c->probe_cap = 0;
   c->probe_steps = 0;
   if (count == 0) return DG3D_OK;
-  for (int i = 0; i < count; ++i)
-    if (nodes[i] < 0 || nodes[i] >= c->fs) return c->fail(DG3D_E_INVALID, "probe node out of range");
+  std::vector<int64_t> slots(count);
+  for (int i = 0; i < count; ++i) {
+    if (nodes[i] < 0 || nodes[i] >= c->E * c->np) return c->fail(DG3D_E_INVALID, "probe node out of range");
+    slots[i] = (nodes[i] / c->np) * c->es + nodes[i] % c->np;  // host numbering -> device slot
+  }
   if (dmalloc(&c->d_probe_nodes, count) != hipSuccess ||
       dmalloc(&c->d_probe_buf, (size_t)count * std::max(capacity_steps, 1)) != hipSuccess)
     return c->fail(DG3D_E_NOMEM, "probe allocation failed");
-  hipMemcpy(c->d_probe_nodes, nodes, sizeof(int64_t) * count, hipMemcpyHostToDevice);
+  hipMemcpy(c->d_probe_nodes, slots.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice);
   c->probe_count = count;
   c->probe_cap = capacity_steps;
   return DG3D_OK;

@@ -34,6 +34,17 @@ __host__ __device__ inline uint32_t div32(uint32_t n, Div32 d) {
   return (uint32_t)(((uint64_t)n * d.m) >> (31 + d.s));
 }
 
+// Device layout of a field: element-major, with ES(n) node slots per element. n^3 is rounded
+// up to a multiple of 32 slots (n = 3, 5, 6, 7), so every element of an FP64 or FP32 field
+// starts on a 128-byte line and a workgroup's elements cover whole lines: no line is written
+// in part by two workgroups. The padding slots hold zeros. n = 2 keeps 8 slots: its
+// workgroups take 16 or 32 elements, which are whole lines too. A field has fs =
+// round_up(E * ES, 32) slots.
+constexpr int elem_stride(int n) {
+  return (n == 2 || (n * n * n) % 32 == 0) ? n * n * n : (n * n * n + 31) / 32 * 32;
+}
+inline int64_t field_slots(int64_t E, int n) { return (E * elem_stride(n) + 31) / 32 * 32; }
+
 constexpr int kMatStride = 8;  // per-element material record (doubles)
 enum MatSlot { MAT_C = 0, MAT_RHO0 = 1, MAT_BA = 2, MAT_SIGMA = 3, MAT_IRHO0 = 4 };
 
@@ -41,7 +52,8 @@ struct MeshArgs {
   int nE[3];
   int periodic[3];
   int64_t E;          // interior elements (< 2^31)
-  int64_t fs;         // field stride in a register = E * n^3
+  int64_t fs;         // field stride in a register: field_slots(E, n)
+  int es;             // node slots per element: elem_stride(n)
   int64_t stride[3];  // element index stride per axis: e = (i*nE1 + j)*nE2 + k
   Div32 div12;        // divide by nE1*nE2
   Div32 div2;         // divide by nE2

@@ -26,11 +26,12 @@ struct Geo {
   static constexpr int NP = NN * NN * NN;
   static constexpr int NF2 = NN * NN;
   static constexpr int EPB = EPB_;
-  static constexpr int NT = ((EPB * NP + 63) / 64) * 64;
+  static constexpr int ES = elem_stride(NN);  // node slots per element in a field array
+  static constexpr int NT = ((EPB * ES + 63) / 64) * 64;
 };
 // Default shape per order (whole waves with < 10% idle lanes); used by the LDG kernel.
 template <int NN>
-struct Shape : Geo<NN, (NN == 2) ? 32 : (NN == 3) ? 9 : (NN == 4) ? 4 : (NN == 5) ? 2 : (NN == 6) ? 2 : (NN == 7) ? 2 : 1> {};
+struct Shape : Geo<NN, (NN == 2) ? 32 : (NN == 3) ? 8 : (NN == 4) ? 4 : (NN == 5) ? 2 : (NN == 6) ? 2 : (NN == 7) ? 2 : 1> {};
 
 // Tunable variants of the stage kernel (chosen at run time by the autotuner of the C ABI):
 // bit 0 selects the alternative elements-per-workgroup, bit 1 the occupancy target (minimum
@@ -38,8 +39,8 @@ struct Shape : Geo<NN, (NN == 2) ? 32 : (NN == 3) ? 9 : (NN == 4) ? 4 : (NN == 5
 template <int NN>
 struct Variants {
   static constexpr int EPB0 = Shape<NN>::EPB;
-  static constexpr int EPB1 = (NN == 2) ? 16 : (NN == 3) ? 7 : (NN == 4) ? 8 : (NN == 5) ? 4 : (NN == 8) ? 2 : 1;
-  static constexpr int MINW1 = (NN <= 5) ? 8 : 6;
+  static constexpr int EPB1 = (NN == 2) ? 16 : (NN == 3) ? 16 : (NN == 4) ? 8 : (NN == 5) ? 4 : (NN == 8) ? 2 : 1;
+  static constexpr int MINW1 = 6;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so that XCD x
@@ -108,33 +109,38 @@ __device__ __forceinline__ int face_node(int axis, int ia, int b, int c) {
   return (b * NN + c) * NN + ia;
 }
 
+// Buffer-resource access. One resource per field array (base, size in bytes): the loads and
+// stores of all fields then share one 32-bit offset register, and an access beyond the array
+// reads 0 / is dropped. The context guarantees fewer than 2^29 nodes per field, so byte
+// offsets fit in 32 bits.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr int kPolicyNT = 2;  // nontemporal
+__device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
+  // dword 3 = 0x00020000: 32-bit data format, raw (stride 0) addressing
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T, int POL = 0>
+__device__ __forceinline__ T bload(Rsrc r, uint32_t i) {
+  if constexpr (std::is_same<T, double>::value) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8u, 0, POL));
+  } else {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4u, 0, POL));
+  }
+}
+template <typename T, int POL = 0>
+__device__ __forceinline__ void bstore(T v, Rsrc r, uint32_t i) {
+  if constexpr (std::is_same<T, double>::value) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, i * 8u, 0, POL);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, i * 4u, 0, POL);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
 template <typename T>
 __device__ __forceinline__ void st_stream(T v, T* p) { __builtin_nontemporal_store(v, p); }
-
-// Face work items of a workgroup, (element, direction, face node) in that order: the same for
-// every workgroup, so they are tabulated once. x = element | dir << 6 | face node << 9,
-// y = self node | neighbour node << 16.
-template <int NN, int EPB>
-struct ItemTable {
-  static constexpr int NITEMS = EPB * 6 * NN * NN;
-  unsigned x[NITEMS];
-  unsigned y[NITEMS];
-  constexpr ItemTable() : x(), y() {
-    for (int f = 0; f < NITEMS; ++f) {
-      const int fle = f / (6 * NN * NN), r = f % (6 * NN * NN), dir = r / (NN * NN), fn = r % (NN * NN);
-      const int axis = 2 - dir / 2, side = dir % 2, b = fn / NN, c = fn % NN;
-      const int ia = side ? NN - 1 : 0, io = side ? 0 : NN - 1;
-      const int sn = axis == 0 ? (ia * NN + b) * NN + c : (axis == 1 ? (b * NN + ia) * NN + c : (b * NN + c) * NN + ia);
-      const int on = axis == 0 ? (io * NN + b) * NN + c : (axis == 1 ? (b * NN + io) * NN + c : (b * NN + c) * NN + io);
-      x[f] = (unsigned)fle | ((unsigned)dir << 6) | ((unsigned)fn << 9);
-      y[f] = (unsigned)sn | ((unsigned)on << 16);
-    }
-  }
-};
-template <int NN, int EPB>
-__device__ constexpr ItemTable<NN, EPB> kItems{};
 
 // Per (element, face) flux coefficients. With mom = P/rho0 and mass = m_axis of self (s) and
 // neighbour (o), the upwind pair of Element.cpp:540-600 scaled by the lift factors of
@@ -314,6 +320,19 @@ __device__ __forceinline__ T sel3(int a, T x0, T x1, T x2) {
   return a == 0 ? x0 : (a == 1 ? x1 : x2);
 }
 
+// Across face `dir` of element he with coordinates ec: the neighbour element (periodic wrap
+// included), or -1 - (index of he among the block's elements on that face) for the exterior.
+__device__ __forceinline__ int face_neighbour(const MeshArgs& m, int he, const int* ec, int dir) {
+  const int axis = 2 - (dir >> 1);
+  const int pos = sel3(axis, ec[0], ec[1], ec[2]);
+  const int ne = sel3(axis, m.nE[0], m.nE[1], m.nE[2]);
+  const int st = (int)sel3(axis, m.stride[0], m.stride[1], m.stride[2]);
+  const int per = sel3(axis, m.periodic[0], m.periodic[1], m.periodic[2]);
+  const int fe = sel3(axis, ec[1] * m.nE[2] + ec[2], ec[0] * m.nE[2] + ec[2], ec[0] * m.nE[1] + ec[1]);
+  if (dir & 1) return (pos < ne - 1) ? he + st : (per ? he - (ne - 1) * st : -1 - fe);
+  return (pos > 0) ? he - st : (per ? he + (ne - 1) * st : -1 - fe);
+}
+
 template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT, int CONTRACT = 0>
 __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunch L) {
   using S = Geo<NN, EPB>;
@@ -323,7 +342,6 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
   __shared__ CT sF[EPB * 4 * NP];                  // P/rho0, m_x, m_y, m_z per node
   __shared__ CT sFl[EPB * 6 * NF2 * 2];            // scaled flux pair per face node
   __shared__ CT sK[EPB * 6 * kNK];                 // flux coefficients per face
-  __shared__ int sNb[EPB * 6];                     // neighbour element, or -1 - face element index
   __shared__ CT sIw[EPB * 3];                      // 1/width per element and axis
   __shared__ float sOut[CONTRACT ? EPB * 6 * NP : 1];  // matrix-core contraction results
   __shared__ CT sA[NN * NN];
@@ -337,79 +355,77 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
   const int E = (int)m.E;
   const int64_t fs = m.fs;
 
-  // ---- phase 0: the node's own loads; helper lanes find each face's neighbour ----------
-  const int le = tid / NP;
-  const int node = tid - le * NP;
+  // ---- phase 0: issue every global load of the workgroup -------------------------------
+  // The node's state and RK register, the neighbour traces of the lane's face items, the
+  // material on both sides of each face, the line operators and 1/width. Neighbours are found
+  // by index arithmetic, so no load waits on another and all are in flight together.
+  // lane -> (element, slot); slots n^3..ES-1 are padding: loaded and stored (as zeros) so the
+  // workgroup writes whole lines, but they take no part in the operator
+  constexpr int ES = S::ES;
+  const int le = tid / ES;
+  const int node = tid - le * ES;
   const int e = e0 + le;
-  const bool valid = (le < EPB) && (e < E);
-  const int64_t gi = (int64_t)(valid ? e : 0) * NP + node;
+  const bool io = (le < EPB) && (e < E);
+  const bool valid = io && node < NP;
+  const uint32_t gi = (uint32_t)(io ? e : 0) * ES + node;
+  const uint32_t fb64 = (uint32_t)(fs * 8), fbdu = (uint32_t)(fs * sizeof(DUT));
+  Rsrc rq[4], rp[4];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) rq[f] = make_rsrc(L.q_in + f * fs, fb64);
   double q[4], pre[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int f = 0; f < 4; ++f) q[f] = L.q_in[f * fs + gi];
-  if (EPI == EPI_LSRK && L.a != 0.0) {
-    const DUT* dup = static_cast<const DUT*>(L.du);
+  for (int f = 0; f < 4; ++f) q[f] = bload<double>(rq[f], gi);
+  if constexpr (EPI == EPI_LSRK) {
 #pragma unroll
-    for (int f = 0; f < 4; ++f) pre[f] = (double)ld_stream(dup + f * fs + gi);
-  }
-  if (EPI == EPI_RK33 && L.a != 0.0) {
+    for (int f = 0; f < 4; ++f) rp[f] = make_rsrc(static_cast<const DUT*>(L.du) + f * fs, fbdu);
+    if (L.a != 0.0) {
 #pragma unroll
-    for (int f = 0; f < 4; ++f) pre[f] = ld_stream(L.old + f * fs + gi);
-  }
-  const double* mp = m.mat + (int64_t)(valid ? e : 0) * kMatStride;
-  const CT c = (CT)mp[MAT_C], r0 = (CT)mp[MAT_RHO0], ba = (CT)mp[MAT_BA], sg = (CT)mp[MAT_SIGMA];
-  const CT ir0 = (CT)mp[MAT_IRHO0];
-  if (tid < EPB * 6) {
-    const int hle = tid / 6, dir = tid - hle * 6;
-    const int he = e0 + hle;
-    int nb = -1;
-    if (he < E) {
-      const int axis = 2 - (dir >> 1);
-      int ec[3];
-      elem_coords(m, (uint32_t)he, ec);
-      const int pos = sel3(axis, ec[0], ec[1], ec[2]);
-      const int ne = sel3(axis, m.nE[0], m.nE[1], m.nE[2]);
-      const int st = (int)sel3(axis, m.stride[0], m.stride[1], m.stride[2]);
-      const int per = sel3(axis, m.periodic[0], m.periodic[1], m.periodic[2]);
-      const int fe = sel3(axis, ec[1] * m.nE[2] + ec[2], ec[0] * m.nE[2] + ec[2], ec[0] * m.nE[1] + ec[1]);
-      if (dir & 1) nb = (pos < ne - 1) ? he + st : (per ? he - (ne - 1) * st : -1 - fe);
-      else nb = (pos > 0) ? he - st : (per ? he + (ne - 1) * st : -1 - fe);
-      if ((dir & 1) == 0) sIw[hle * 3 + axis] = (CT)sel3(axis, m.inv_w[0], m.inv_w[1], m.inv_w[2])[pos];
-    }
-    sNb[tid] = nb;
-  }
-  if (tid < NN * NN) {
-    if constexpr (std::is_same<CT, double>::value) sA[tid] = L.opd[0][tid];
-    else sA[tid] = L.opf[0][tid];
-  }
-  if (tid < NN) {
-    if constexpr (std::is_same<CT, double>::value) {
-      sL0[tid] = L.opd[1][tid];
-      sLN[tid] = L.opd[2][tid];
-    } else {
-      sL0[tid] = L.opf[1][tid];
-      sLN[tid] = L.opf[2][tid];
+      for (int f = 0; f < 4; ++f) pre[f] = (double)bload<DUT, kPolicyNT>(rp[f], gi);
     }
   }
-  __syncthreads();
+  if constexpr (EPI == EPI_RK33) {
+    if (L.a != 0.0) {
+#pragma unroll
+      for (int f = 0; f < 4; ++f) pre[f] = bload<double, kPolicyNT>(make_rsrc(L.old + f * fs, fb64), gi);
+    }
+  }
+  const Rsrc rm = make_rsrc(m.mat, (uint32_t)(m.E * kMatStride * 8));
+  const uint32_t mi = (uint32_t)(valid ? e : 0) * kMatStride;
+  const CT c = (CT)bload<double>(rm, mi + MAT_C), r0 = (CT)bload<double>(rm, mi + MAT_RHO0);
+  const CT ba = (CT)bload<double>(rm, mi + MAT_BA), sg = (CT)bload<double>(rm, mi + MAT_SIGMA);
+  const CT ir0 = (CT)bload<double>(rm, mi + MAT_IRHO0);
 
-  // ---- phase 1: neighbour-trace loads, flux coefficients, the node's kernel functions ----
+  // face items (element, direction, face node) of this lane; item = fle | dir << 6 | fn << 9 |
+  // self node << 16, or ~0u when the item has nothing to do
   double nrho[R], nu[R];
+  unsigned item[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int f = tid + r * NT;
     nrho[r] = 0.0;
     nu[r] = 0.0;
+    item[r] = ~0u;
     if (f < NITEMS) {
-      const unsigned ix = kItems<NN, EPB>.x[f], iy = kItems<NN, EPB>.y[f];
-      const int fle = ix & 63u, dir = (ix >> 6) & 7u, fn = ix >> 9;
-      const int on = iy >> 16;
+      const int fle = f / (6 * NF2);
+      const int rr = f - fle * (6 * NF2);
+      const int dir = rr / NF2;
+      const int fn = rr - dir * NF2;
       const int axis = 2 - (dir >> 1);
-      if (e0 + fle < E && ((mask >> (2 * axis)) & 3u) != 0u) {
-        const int nb = sNb[fle * 6 + dir];
+      const int he = e0 + fle;
+      if (he < E && ((mask >> (2 * axis)) & 3u) != 0u) {
+        const int b = fn / NN, cc = fn - (fn / NN) * NN;
+        const int sn = face_node<NN>(axis, (dir & 1) ? NN - 1 : 0, b, cc);
+        const int on = face_node<NN>(axis, (dir & 1) ? 0 : NN - 1, b, cc);
+        item[r] = (unsigned)fle | ((unsigned)dir << 6) | ((unsigned)fn << 9) | ((unsigned)sn << 16);
+        int ec[3];
+        elem_coords(m, (uint32_t)he, ec);
+        const int nb = face_neighbour(m, he, ec, dir);
         if (nb >= 0) {
-          const int64_t oi = (int64_t)nb * NP + on;
-          nrho[r] = L.q_in[oi];
-          nu[r] = L.q_in[(1 + axis) * fs + oi];
+          const uint32_t oi = (uint32_t)nb * ES + on;
+          nrho[r] = bload<double>(rq[0], oi);
+          if (axis == 0) nu[r] = bload<double>(rq[1], oi);
+          else if (axis == 1) nu[r] = bload<double>(rq[2], oi);
+          else nu[r] = bload<double>(rq[3], oi);
         } else {
           const double* tr = L.ext.trace[0];
           if (dir == 1) tr = L.ext.trace[1];
@@ -426,23 +442,45 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
       }
     }
   }
-  if (tid < EPB * 6) {
-    const int hle = tid / 6, dir = tid - hle * 6;
-    const int he = e0 + hle;
-    CT K[kNK];
-#pragma unroll
-    for (int k = 0; k < kNK; ++k) K[k] = CT(0);
-    if (he < E) {
-      const int nb = sNb[tid];
-      const double* ms = m.mat + (int64_t)he * kMatStride;
-      // the halo element has the adjacent element's material (Block.cpp:167-248)
-      const double* mo = m.mat + (int64_t)(nb >= 0 ? nb : he) * kMatStride;
-      face_coeffs<CT>(dir & 1, (CT)ms[MAT_C], (CT)ms[MAT_RHO0], (CT)ms[MAT_IRHO0], (CT)mo[MAT_C],
-                      (CT)mo[MAT_RHO0], (CT)mo[MAT_BA], (CT)mo[MAT_IRHO0], K);
-    }
-#pragma unroll
-    for (int k = 0; k < kNK; ++k) sK[tid * kNK + k] = K[k];
+
+  // helper lanes, one per (element, face): material of both sides and 1/width
+  const bool helper = tid < EPB * 6;
+  const int hle = tid / 6, hdir = tid - (tid / 6) * 6;
+  const int he = e0 + hle;
+  double ms_c = 1.0, ms_r = 1.0, ms_ir = 1.0, mo_c = 1.0, mo_r = 1.0, mo_ba = 0.0, mo_ir = 1.0, hiw = 0.0;
+  if (helper && he < E) {
+    int ec[3];
+    elem_coords(m, (uint32_t)he, ec);
+    const int nb = face_neighbour(m, he, ec, hdir);
+    const uint32_t ms = (uint32_t)he * kMatStride;
+    // the halo element has the adjacent element's material (Block.cpp:167-248)
+    const uint32_t mo = (uint32_t)(nb >= 0 ? nb : he) * kMatStride;
+    ms_c = bload<double>(rm, ms + MAT_C);
+    ms_r = bload<double>(rm, ms + MAT_RHO0);
+    ms_ir = bload<double>(rm, ms + MAT_IRHO0);
+    mo_c = bload<double>(rm, mo + MAT_C);
+    mo_r = bload<double>(rm, mo + MAT_RHO0);
+    mo_ba = bload<double>(rm, mo + MAT_BA);
+    mo_ir = bload<double>(rm, mo + MAT_IRHO0);
+    const int axis = 2 - (hdir >> 1);
+    if ((hdir & 1) == 0) hiw = sel3(axis, m.inv_w[0], m.inv_w[1], m.inv_w[2])[sel3(axis, ec[0], ec[1], ec[2])];
   }
+  CT av = CT(0), l0v = CT(0), lnv = CT(0);
+  if (tid < NN * NN) {
+    if constexpr (std::is_same<CT, double>::value) av = L.opd[0][tid];
+    else av = L.opf[0][tid];
+  }
+  if (tid < NN) {
+    if constexpr (std::is_same<CT, double>::value) {
+      l0v = L.opd[1][tid];
+      lnv = L.opd[2][tid];
+    } else {
+      l0v = L.opf[1][tid];
+      lnv = L.opf[2][tid];
+    }
+  }
+
+  // ---- phase 1: kernel functions, flux coefficients and operators into LDS --------------
   if (valid) {
     const CT rho = (CT)q[0];
     CT* F = sF + le * 4 * NP;
@@ -450,18 +488,32 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
 #pragma unroll
     for (int a = 0; a < 3; ++a) F[(1 + a) * NP + node] = mass_f((CT)q[1 + a], rho, r0, ba);
   }
+  if (helper) {
+    CT K[kNK];
+#pragma unroll
+    for (int k = 0; k < kNK; ++k) K[k] = CT(0);
+    if (he < E) {
+      face_coeffs<CT>(hdir & 1, (CT)ms_c, (CT)ms_r, (CT)ms_ir, (CT)mo_c, (CT)mo_r, (CT)mo_ba, (CT)mo_ir, K);
+      if ((hdir & 1) == 0) sIw[hle * 3 + 2 - (hdir >> 1)] = (CT)hiw;
+    }
+#pragma unroll
+    for (int k = 0; k < kNK; ++k) sK[tid * kNK + k] = K[k];
+  }
+  if (tid < NN * NN) sA[tid] = av;
+  if (tid < NN) {
+    sL0[tid] = l0v;
+    sLN[tid] = lnv;
+  }
   __syncthreads();
 
   // ---- phase 2: upwind flux pair of every face node -------------------------------------
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int f = tid + r * NT;
-    if (f >= NITEMS) continue;
-    const unsigned ix = kItems<NN, EPB>.x[f], iy = kItems<NN, EPB>.y[f];
-    const int fle = ix & 63u, dir = (ix >> 6) & 7u, fn = ix >> 9;
-    const int sn = iy & 0xFFFFu;
+    const unsigned it = item[r];
+    if (it == ~0u) continue;
+    const int fle = it & 63u, dir = (it >> 6) & 7u, fn = (it >> 9) & 127u;
+    const int sn = it >> 16;
     const int axis = 2 - (dir >> 1);
-    if (e0 + fle >= E || ((mask >> (2 * axis)) & 3u) == 0u) continue;
     const CT* F = sF + fle * 4 * NP;
     const CT mom_s = F[sn];
     const CT mass_s = F[(1 + axis) * NP + sn];
@@ -479,7 +531,7 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
     __syncthreads();
   }
 
-  if (!valid) return;
+  if (!io) return;
 
   // ---- phase 3: line contractions, lifting, sponge, RK update ---------------------------
   const int idx[3] = {node / NF2, (node / NN) % NN, node % NN};
@@ -490,7 +542,7 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const unsigned am = (mask >> (2 * a)) & 3u;
-    if (am == 0u) continue;
+    if (am == 0u || !valid) continue;
     const int ix = idx[a];
     const int ls = (a == 0) ? NF2 : (a == 1 ? NN : 1);
     const int base = node - ix * ls;
@@ -529,24 +581,27 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
 #pragma unroll
     for (int a = 0; a < 3; ++a) touch[1 + a] = (mask & (kTermSponge | (1u << (2 * a)))) != 0u;
 #pragma unroll
-    for (int f = 0; f < 4; ++f)
-      if (touch[f]) L.out[f * fs + gi] += (double)rhs[f];
+    for (int f = 0; f < 4; ++f) {
+      if (touch[f]) {
+        const Rsrc ro = make_rsrc(L.out + f * fs, fb64);
+        bstore<double>(bload<double>(ro, gi) + (double)rhs[f], ro, gi);
+      }
+    }
   } else if constexpr (EPI == EPI_LSRK) {
-    DUT* dup = static_cast<DUT*>(L.du);
     if constexpr (std::is_same<DUT, double>::value) {
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
         const double dn = L.a * pre[f] + L.dt * (double)rhs[f];
-        st_stream(dn, dup + f * fs + gi);
-        st_stream(q[f] + L.b * dn, L.q_out + f * fs + gi);
+        bstore<double, kPolicyNT>(dn, rp[f], gi);
+        bstore<double, kPolicyNT>(q[f] + L.b * dn, make_rsrc(L.q_out + f * fs, fb64), gi);
       }
     } else {
       const float af = (float)L.a, dtf = (float)L.dt;
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
         const float dn = af * (float)pre[f] + dtf * (float)rhs[f];
-        st_stream(dn, dup + f * fs + gi);
-        st_stream(q[f] + L.b * (double)dn, L.q_out + f * fs + gi);
+        bstore<float, kPolicyNT>(dn, rp[f], gi);
+        bstore<double, kPolicyNT>(q[f] + L.b * (double)dn, make_rsrc(L.q_out + f * fs, fb64), gi);
       }
     }
   } else {
@@ -556,7 +611,7 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
     for (int f = 0; f < 4; ++f) {
       double v = (double)rhs[f] * sb + q[f] * L.b;
       if (L.a != 0.0) v += pre[f] * L.a;
-      L.out[f * fs + gi] = v;
+      bstore<double>(v, make_rsrc(L.out + f * fs, fb64), gi);
     }
   }
 }
@@ -602,7 +657,7 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
   const int node = tid - le * NP;
   const int e = e0 + le;
   const bool valid = (le < EPB) && (e < (int)m.E);
-  const int64_t gi = (int64_t)e * NP + node;
+  const int64_t gi = (int64_t)e * S::ES + node;
   const int axis = L.axis;
   const int fin = (L.which == 0) ? 0 : 4 + axis;  // rho' or Q_axis
   if (valid) sV[le * NP + node] = L.in[fin * m.fs + gi];
@@ -624,7 +679,7 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
     const int eo = neighbour(m, fe, ec, axis, side);
     const int dir = (axis == 0) ? 4 + side : (axis == 1 ? 2 + side : side);
     if (eo >= 0) {
-      vo = L.in[fin * m.fs + (int64_t)eo * NP + on];
+      vo = L.in[fin * m.fs + (int64_t)eo * S::ES + on];
     } else {
       const int64_t fidx = face_elem(m, ec, axis);
       const double* tr = L.ext.trace[dir];
@@ -718,11 +773,12 @@ __global__ void __launch_bounds__(kRedThreads) k_l2diff(const double* a, const d
 __global__ void __launch_bounds__(kRedThreads) k_energy(const double* reg, MeshArgs m, int n, const double* wq, double* part) {
   __shared__ double sh[kRedThreads / 64];
   auto add = [](double x, double y) { return x + y; };
-  const int64_t np = (int64_t)n * n * n;
+  const int np = n * n * n;
   double en = 0.0, ms = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m.fs; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t e = i / np;
-    const int nd = (int)(i - e * np);
+    const int64_t e = i / m.es;
+    const int nd = (int)(i - e * m.es);
+    if (e >= m.E || nd >= np) continue;  // padding slots
     const int a = nd / (n * n), b = (nd / n) % n, c = nd % n;
     int ec[3];
     elem_coords(m, (uint32_t)e, ec);
@@ -760,9 +816,8 @@ __global__ void __launch_bounds__(kRedThreads) k_cfl(MeshArgs m, double dxi_half
 __global__ void k_probe(const double* reg, MeshArgs m, int n, const int64_t* nodes, int count, double* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
-  const int64_t g = nodes[i];
-  const int64_t np = (int64_t)n * n * n;
-  const int64_t e = g / np;
+  const int64_t g = nodes[i];  // device slot
+  const int64_t e = g / m.es;
   const double c = m.mat[e * kMatStride + MAT_C], r0 = m.mat[e * kMatStride + MAT_RHO0];
   const double ba = m.mat[e * kMatStride + MAT_BA];
   const double r = reg[g];
