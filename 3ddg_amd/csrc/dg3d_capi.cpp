@@ -286,7 +286,7 @@ int autotune(dg3d_ctx* c, int scheme) {
     if (e != hipSuccess) return c->hip_fail(e, "autotune");
   }
   for (int v = 0; v < dg3d::kNumVariants; ++v) {
-    for (int x = 0; x <= 1; ++x) {
+    for (int x = 0; x < dg3d::kNumOrders; ++x) {
       dg3d::RhsLaunch L = base_launch(c);
       L.variant = v;
       L.xcd_remap = x;
@@ -344,8 +344,9 @@ int dg3d_autotune(dg3d_ctx* c, int scheme, int* variant, int* xcd_remap) {
 int dg3d_set_variant(dg3d_ctx* c, int scheme, int variant, int xcd_remap) {
   if (!c || (scheme != DG3D_LSRK54 && scheme != DG3D_RK33)) return c ? c->fail(DG3D_E_INVALID, "set_variant: bad scheme") : DG3D_E_INVALID;
   if (variant < 0 || variant >= dg3d::kNumVariants) return c->fail(DG3D_E_INVALID, "set_variant: variant out of range");
+  if (xcd_remap < 0 || xcd_remap >= dg3d::kNumOrders) return c->fail(DG3D_E_INVALID, "set_variant: order out of range");
   c->tune_variant[scheme] = variant;
-  c->tune_xcd[scheme] = xcd_remap ? 1 : 0;
+  c->tune_xcd[scheme] = xcd_remap;
   return DG3D_OK;
 }
 
@@ -394,7 +395,7 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
     c->nfields = d->with_q ? 7 : 4;
     c->precision = d->precision;
     c->contraction = d->contraction;
-    c->xcd = d->xcd_remap;
+    c->xcd = (d->xcd_remap >= 0 && d->xcd_remap < dg3d::kNumOrders) ? d->xcd_remap : 0;
     c->nodes.assign(d->lgl_nodes, d->lgl_nodes + c->n);
     c->weights.assign(d->lgl_weights, d->lgl_weights + c->n);
     dg3d::LineOperator op;

@@ -81,7 +81,7 @@ struct RhsLaunch {
   int mixed;        // 0: FP64 compute, 1: FP32 compute
   int epilogue;     // Epilogue
   unsigned mask;    // term mask
-  int xcd_remap;
+  int xcd_remap;    // workgroup order 0..kNumOrders-1 (autotuned)
   int variant;      // kernel shape variant 0..kNumVariants-1 (autotuned)
   int contraction;  // mixed path: 0 VALU, 1 MFMA FP32, 2 MFMA split-BF16
   MeshArgs mesh;
@@ -98,6 +98,7 @@ struct RhsLaunch {
 
 hipError_t launch_rhs(const RhsLaunch& L, hipStream_t s);
 constexpr int kNumVariants = 4;
+constexpr int kNumOrders = 4;
 
 // Exterior trace of the source face at stage time x.t (halo trace + boundary source) into `out`
 // ([faceElem][n*n][3]); the RHS kernel then reads it as that face's halo.

@@ -43,13 +43,55 @@ struct Variants {
   static constexpr int MINW1 = 6;
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs; renumber them so that XCD x
-// processes one contiguous range of elements and its L2 sees the neighbour traces.
-__device__ __forceinline__ int64_t remap_block(int64_t w, int64_t G, int on) {
-  if (!on) return w;
-  const int64_t X = 8;
-  const int64_t per = G / X, rem = G % X, x = w % X, i = w / X;
-  return x * per + (x < rem ? x : rem) + i;
+// Workgroup order. The hardware dispatches workgroup w to XCD w mod 8, and each XCD has its own
+// L2, so the order decides which neighbour traces an XCD finds in its L2:
+//   0: hardware order (XCD x takes groups x, x+8, ...);
+//   1: XCD x takes one contiguous slab of G/8 groups;
+//   2: XCD x takes runs of C consecutive groups (runs x, x+8, ... in time), with C chosen so
+//      that the groups one element row (y) and one element plane (x) away fall on the same XCD
+//      while the 8 XCDs stay in one compact window of memory.
+__device__ __forceinline__ int64_t remap_block(int64_t w, int64_t G, int mode, int64_t C) {
+  constexpr int64_t X = 8;
+  if (mode == 1) {
+    const int64_t per = G / X, rem = G % X, x = w % X, i = w / X;
+    return x * per + (x < rem ? x : rem) + i;
+  }
+  if (mode == 2) {
+    const int64_t full = G / (X * C) * (X * C);
+    if (w >= full) return w;
+    const int64_t x = w % X, i = w / X;
+    return (i / C) * (X * C) + x * C + (i % C);
+  }
+  return w;
+}
+// Order 3: order 1's XCD slabs over a tiled traversal of the element groups. With Kg = nE2/EPB
+// groups per (i, j) column, the traversal runs over blocks of TJ element rows (y) and TK groups
+// (z); inside a block i is outermost, then j, then the group: the groups one plane (x), one row
+// (y) and one element (z) away are at most TJ*TK, TK and 1 positions apart, so an XCD finds
+// most neighbour traces in its L2. Falls back to order 1 when the mesh does not divide.
+constexpr int kTileJ = 8, kTileK = 8;
+__device__ __forceinline__ int64_t tiled_group(int64_t s, int nE0, int nE1, int nE2, int epb) {
+  if (nE2 % epb != 0) return s;
+  const int kg = nE2 / epb;
+  const int tk = kg < kTileK ? kg : kTileK, tj = nE1 < kTileJ ? nE1 : kTileJ;
+  if (kg % tk != 0 || nE1 % tj != 0) return s;
+  const int64_t per_jb = (int64_t)tj * nE0 * kg, per_kb = (int64_t)nE0 * tj * tk;
+  const int64_t jb = s / per_jb;
+  int64_t r = s - jb * per_jb;
+  const int64_t kb = r / per_kb;
+  r -= kb * per_kb;
+  const int64_t i = r / (tj * tk);
+  r -= i * (tj * tk);
+  const int64_t jj = r / tk, kk = r - (r / tk) * tk;
+  return (i * nE1 + jb * tj + jj) * kg + kb * tk + kk;
+}
+
+// Run length of order 2 for EPB elements per group: a y-row of nE2/EPB groups split over 8 XCDs.
+__device__ __forceinline__ int64_t order_run(int nE2, int epb) {
+  if (nE2 % epb != 0) return 4;
+  const int sy = nE2 / epb;
+  if (sy % 8 == 0) return sy / 8;
+  return sy;
 }
 
 // kernelFuncs.cpp:17-20: P/rho0 = (rho' + rho'^2 * (B/A) * 0.5 / rho0) * c^2 / rho0
@@ -351,7 +393,12 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
   const MeshArgs& m = L.mesh;
   const unsigned mask = L.mask;
   const int tid = threadIdx.x;
-  const int e0 = (int)(remap_block(blockIdx.x, gridDim.x, L.xcd_remap) * EPB);
+  int64_t grp;
+  if (L.xcd_remap == 3)
+    grp = tiled_group(remap_block(blockIdx.x, gridDim.x, 1, 1), m.nE[0], m.nE[1], m.nE[2], EPB);
+  else
+    grp = remap_block(blockIdx.x, gridDim.x, L.xcd_remap, order_run(m.nE[2], EPB));
+  const int e0 = (int)(grp * EPB);
   const int E = (int)m.E;
   const int64_t fs = m.fs;
 

@@ -16,7 +16,7 @@ Solver::Solver(const Mesh& mesh, const int* periodic, int precision, int registe
   d.contraction = DG3D_CONTRACT_VALU;
   d.num_registers = registers;
   d.with_q = with_q ? 1 : 0;
-  d.xcd_remap = 1;
+  d.xcd_remap = 2;
   d.mass = mesh.mass.data();
   d.stiff = mesh.stiff.data();
   d.lgl_nodes = mesh.nodes.data();

@@ -169,7 +169,7 @@ class Context:
     """One block's device state (the reference's Grid/Block for one rank)."""
 
     def __init__(self, order, num_elements, mass, stiff, nodes, weights, periodic=(0, 0, 0), precision=FP64,
-                 contraction=CONTRACT_VALU, num_registers=1, with_q=False, xcd_remap=True):
+                 contraction=CONTRACT_VALU, num_registers=1, with_q=False, xcd_remap=2):
         self._L = lib()
         self.order = order
         self.n = order + 1
@@ -186,7 +186,7 @@ class Context:
         d.contraction = contraction
         d.num_registers = num_registers
         d.with_q = int(bool(with_q))
-        d.xcd_remap = int(bool(xcd_remap))
+        d.xcd_remap = int(xcd_remap)
         d.mass, d.stiff, d.lgl_nodes, d.lgl_weights = (_ptr(a) for a in self._keep)
         h = ctypes.c_void_p()
         rc = self._L.dg3d_create(ctypes.byref(d), ctypes.byref(h))

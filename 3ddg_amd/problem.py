@@ -299,7 +299,7 @@ def config_c5(num_elements=48, steps=2000):
 
 
 def make_context(p, precision=dg3d.FP64, contraction=dg3d.CONTRACT_VALU, num_registers=1, with_q=False,
-                 xcd_remap=True, halos=True):
+                 xcd_remap=2, halos=True):
     """A device context loaded with the problem: geometry, material, IC (register 0), IC halo
     traces on register 0, and the source."""
     ctx = dg3d.Context(p.order, p.num_elements, p.mass, p.stiff, p.nodes, p.weights, periodic=p.periodic,

@@ -78,7 +78,10 @@ typedef struct dg3d_desc {
   int contraction;         /* DG3D_CONTRACT_* (MIXED only) */
   int num_registers;       /* state registers (Grid copies); >= 1 */
   int with_q;              /* allocate the LDG fields Qx, Qy, Qz (EVALUATE_DIFFUSION) */
-  int xcd_remap;           /* map workgroups so each XCD streams a contiguous slab (1 = on) */
+  int xcd_remap;           /* workgroup order before autotuning: 0 hardware round-robin over the
+                              8 XCDs, 1 one contiguous slab per XCD, 2 runs of consecutive
+                              element groups per XCD sized to the element rows, 3 slabs of a
+                              traversal tiled over 8 rows x 8 groups (L2 reuse of the traces) */
   const double* mass;      /* n*n row-major unit-interval mass M (matrices.txt, Grid.cpp:979) */
   const double* stiff;     /* n*n row-major weak stiffness S (matrices.txt, Grid.cpp:980) */
   const double* lgl_nodes; /* n LGL nodes on [-1,1] (hardcoded_coeff.txt, Basis.cpp:142-153) */
@@ -163,10 +166,11 @@ int dg3d_step(dg3d_ctx* ctx, int scheme, double dt, int nsteps, double* t);
 int dg3d_stage(dg3d_ctx* ctx, int scheme, int stage, double dt, double t);
 
 /* Autotuner of the fused stage kernel (no counterpart in the reference): times every kernel
- * shape variant (elements per workgroup x occupancy target) with and without XCD-contiguous
- * workgroup numbering on side-effect-free stage-0 launches and keeps the fastest for `scheme`.
- * dg3d_step runs it on its first call unless a variant was pinned with dg3d_set_variant.
- * Results do not depend on the variant (the per-node arithmetic is the same). */
+ * shape variant (elements per workgroup x occupancy target, 0..3) under every workgroup order
+ * (0..3, as xcd_remap above) on launches with a later stage's coefficients that write only the
+ * work registers, and keeps the fastest pair for `scheme`. dg3d_step runs it on its first call
+ * unless a pair was pinned with dg3d_set_variant. With the VALU contraction the results do not
+ * depend on the pair (the per-node arithmetic is the same). */
 int dg3d_autotune(dg3d_ctx* ctx, int scheme, int* variant, int* xcd_remap);
 int dg3d_set_variant(dg3d_ctx* ctx, int scheme, int variant, int xcd_remap);
 
