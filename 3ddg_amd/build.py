@@ -37,7 +37,7 @@ def build_library(verbose_resources=False):
         _run([HIPCC, "-c", f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, *extra,
               os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    for src in ["dg3d_capi.cpp", "dg3d_setup.cpp", "basis.cpp"]:
+    for src in ["dg3d_capi.cpp", "dg3d_batch.cpp", "dg3d_setup.cpp", "basis.cpp"]:
         obj = os.path.join(objdir, src + ".o")
         _run([HIPCC, "-x", "c++", "-c", *COMMON, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -1,0 +1,526 @@
+// C ABI of the batched path: many independent blocks ("solves") of different element counts and
+// orders, advanced by LSRK54. Per stage, every order present gets one persistent launch over all
+// its solves' element groups; the launches of a stage run as parallel graph branches, and whole
+// steps are captured in HIP graphs and replayed.
+//
+// Each solve is what one dg3d_ctx holds (Grid.cpp:176-375 builds one such block per rank): its
+// own mesh, material, operators and state. The per-node arithmetic is the fused stage of
+// dg3d_step, so a solve of the batch follows the same trajectory as a context of its own.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "basis.hpp"
+#include "dg3d.h"
+#include "dg3d_device.hpp"
+
+namespace {
+
+thread_local std::string g_batch_create_error;
+
+template <typename T>
+hipError_t bmalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * count);
+}
+
+struct HostSolve {
+  int n = 0, np = 0, es = 0;
+  int nE[3] = {0, 0, 0};
+  int periodic[3] = {0, 0, 0};
+  int64_t E = 0, fs = 0, slot0 = 0;
+  double dt = 0.0;
+  bool has_halo[6] = {false, false, false, false, false, false};
+};
+
+struct OrderOps {
+  double* opd[3] = {nullptr, nullptr, nullptr};
+  float* opf[3] = {nullptr, nullptr, nullptr};
+};
+
+}  // namespace
+
+struct dg3d_batch {
+  int precision = DG3D_MIXED;
+  std::vector<HostSolve> solves;
+  std::map<int, OrderOps> ops;         // per order
+  std::vector<double*> allocs;         // per-solve geometry, material and halo arrays
+  double* arena[2] = {nullptr, nullptr};
+  void* du = nullptr;
+  int64_t total_slots = 0;
+  int cur = 0;                         // arena holding the state
+  bool halos_pending = true;           // the next stage is the first of the run
+  dg3d::BatchSolve* d_solves = nullptr;
+  // per order present (descending n): task lists, grids and the tuned shape
+  struct Group {
+    int n = 0;
+    dg3d::BatchTask* d_tasks[dg3d::kNumBatchVariants] = {nullptr, nullptr};
+    int ntasks[dg3d::kNumBatchVariants] = {0, 0};
+    int grid[dg3d::kNumBatchVariants] = {0, 0};
+    int variant = -1;
+    int64_t nodes = 0;
+    hipStream_t stream = nullptr;      // branch of the stage's fork/join
+    hipEvent_t done = nullptr;
+  };
+  std::vector<Group> groups;
+  bool tuned = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // one step starting from arena 0 / arena 1
+  float last_ms = 0.f;
+  int last_launches = 0;
+  std::string err;
+
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    return code;
+  }
+  int hip_fail(hipError_t e, const char* what) {
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return DG3D_E_HIP;
+  }
+};
+
+namespace {
+
+void drop_graphs(dg3d_batch* b) {
+  for (auto& g : b->graph) {
+    if (g) hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+}
+
+void free_batch(dg3d_batch* b) {
+  if (!b) return;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  drop_graphs(b);
+  for (double* p : b->allocs) hipFree(p);
+  for (auto& kv : b->ops) {
+    for (int i = 0; i < 3; ++i) {
+      hipFree(kv.second.opd[i]);
+      hipFree(kv.second.opf[i]);
+    }
+  }
+  hipFree(b->arena[0]);
+  hipFree(b->arena[1]);
+  hipFree(b->du);
+  hipFree(b->d_solves);
+  for (auto& g : b->groups) {
+    for (auto* t : g.d_tasks) hipFree(t);
+    if (g.done) hipEventDestroy(g.done);
+    if (g.stream) hipStreamDestroy(g.stream);
+  }
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+dg3d::BatchLaunch stage_launch(const dg3d_batch* b, const dg3d_batch::Group& g, int variant, int stage, bool halo,
+                               int from) {
+  dg3d::BatchLaunch B{};
+  B.solves = b->d_solves;
+  B.tasks = g.d_tasks[variant];
+  B.ntasks = g.ntasks[variant];
+  B.mixed = (b->precision == DG3D_MIXED) ? 1 : 0;
+  B.use_halo = halo ? 1 : 0;
+  B.q_in = b->arena[from];
+  B.q_out = b->arena[1 - from];
+  B.du = b->du;
+  B.a = dg3d::kLsA[stage];
+  B.b = dg3d::kLsB[stage];
+  return B;
+}
+
+// One stage of every solve: the per-order persistent launches run as parallel branches
+// (fork/join on events), so the orders share the device.
+hipError_t launch_stage(dg3d_batch* b, int stage, bool halo, int from) {
+  hipError_t e = hipEventRecord(b->fork, b->stream);
+  for (auto& g : b->groups) {
+    if (e != hipSuccess) break;
+    e = hipStreamWaitEvent(g.stream, b->fork, 0);
+    if (e == hipSuccess)
+      e = dg3d::launch_rhs_batch(stage_launch(b, g, g.variant, stage, halo, from), g.n, g.variant, g.grid[g.variant],
+                                 g.stream);
+    if (e == hipSuccess) e = hipEventRecord(g.done, g.stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, g.done, 0);
+  }
+  return e;
+}
+
+// One LSRK54 step from arena `from` (5 stages; the state ends in the other arena).
+hipError_t launch_step(dg3d_batch* b, int from, bool first_halo) {
+  int cur = from;
+  for (int st = 0; st < 5; ++st) {
+    hipError_t e = launch_stage(b, st, first_halo && st == 0, cur);
+    if (e != hipSuccess) return e;
+    cur = 1 - cur;
+  }
+  return hipSuccess;
+}
+
+hipError_t build_graphs(dg3d_batch* b) {
+  drop_graphs(b);
+  for (int from = 0; from < 2; ++from) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t le = launch_step(b, from, false);
+    e = hipStreamEndCapture(b->stream, &g);
+    if (le != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      return le;
+    }
+    if (e != hipSuccess) return e;
+    e = hipGraphInstantiate(&b->graph[from], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Per order, times both elements-per-workgroup shapes on a later-stage launch (reads and writes
+// du) that writes only arena 1 - cur and du; du is cleared first, and the run's next stage 0
+// overwrites both.
+int batch_autotune(dg3d_batch* b) {
+  hipError_t e = hipMemsetAsync(b->du, 0, (b->precision == DG3D_MIXED ? 4 : 8) * (size_t)b->total_slots, b->stream);
+  if (e != hipSuccess) return b->hip_fail(e, "batch autotune");
+  for (auto& g : b->groups) {
+    float best = 1e30f;
+    int bv = 0;
+    for (int v = 0; v < dg3d::kNumBatchVariants; ++v) {
+      const dg3d::BatchLaunch B = stage_launch(b, g, v, 1, false, b->cur);
+      e = dg3d::launch_rhs_batch(B, g.n, v, g.grid[v], b->stream);
+      if (e == hipSuccess) e = hipEventRecord(b->ev0, b->stream);
+      for (int r = 0; r < 3 && e == hipSuccess; ++r) e = dg3d::launch_rhs_batch(B, g.n, v, g.grid[v], b->stream);
+      if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
+      if (e != hipSuccess) return b->hip_fail(e, "batch autotune");
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, b->ev0, b->ev1);
+      if (ms < best) {
+        best = ms;
+        bv = v;
+      }
+    }
+    g.variant = bv;
+  }
+  b->tuned = true;
+  return DG3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dg3d_batch_create_error(void) { return g_batch_create_error.c_str(); }
+
+const char* dg3d_batch_last_error(const dg3d_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg3d_batch** out) {
+  if (!out) return DG3D_E_INVALID;
+  *out = nullptr;
+  if (count < 1 || !descs) {
+    g_batch_create_error = "batch: count must be >= 1";
+    return DG3D_E_INVALID;
+  }
+  if (precision != DG3D_FP64 && precision != DG3D_MIXED) {
+    g_batch_create_error = "batch: bad precision";
+    return DG3D_E_INVALID;
+  }
+  dg3d_batch* b = new (std::nothrow) dg3d_batch();
+  if (!b) return DG3D_E_NOMEM;
+  b->precision = precision;
+  auto bail = [&](int code, const std::string& msg) {
+    g_batch_create_error = msg;
+    free_batch(b);
+    return code;
+  };
+  // host-side validation and layout
+  b->solves.resize(count);
+  int64_t slot = 0;
+  for (int i = 0; i < count; ++i) {
+    const dg3d_solve_desc& d = descs[i];
+    HostSolve& h = b->solves[i];
+    const std::string tag = "batch solve " + std::to_string(i) + ": ";
+    if (d.order < 1 || d.order > 7) return bail(DG3D_E_UNSUPPORTED, tag + "order must be 1..7");
+    for (int a = 0; a < 3; ++a) {
+      if (d.num_elements[a] < 1) return bail(DG3D_E_INVALID, tag + "num_elements must be >= 1");
+      if (!d.widths[a]) return bail(DG3D_E_INVALID, tag + "widths missing");
+    }
+    if (!d.mass || !d.stiff || !d.c0 || !d.rho0 || !d.state) return bail(DG3D_E_INVALID, tag + "missing array");
+    if (!(d.dt > 0.0) || !std::isfinite(d.dt)) return bail(DG3D_E_INVALID, tag + "dt must be positive");
+    h.n = d.order + 1;
+    h.np = h.n * h.n * h.n;
+    h.es = dg3d::elem_stride(h.n);
+    for (int a = 0; a < 3; ++a) {
+      h.nE[a] = d.num_elements[a];
+      h.periodic[a] = d.periodic[a] ? 1 : 0;
+    }
+    h.E = (int64_t)h.nE[0] * h.nE[1] * h.nE[2];
+    h.fs = dg3d::field_slots(h.E, h.n);
+    if (h.fs >= (1ll << 29)) return bail(DG3D_E_INVALID, tag + "2^29 or more nodes in one solve");
+    h.slot0 = slot;
+    slot += 4 * h.fs;
+    h.dt = d.dt;
+  }
+  b->total_slots = slot;
+  hipError_t e = hipStreamCreate(&b->stream);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  if (e != hipSuccess) return bail(DG3D_E_HIP, std::string("HIP init: ") + hipGetErrorString(e));
+  // arenas
+  if (bmalloc(&b->arena[0], slot) != hipSuccess || bmalloc(&b->arena[1], slot) != hipSuccess)
+    return bail(DG3D_E_NOMEM, "batch: state arena allocation failed");
+  const size_t du_bytes = (precision == DG3D_MIXED ? 4 : 8) * (size_t)slot;
+  if (hipMalloc(&b->du, du_bytes) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: du arena allocation failed");
+  hipMemset(b->arena[0], 0, sizeof(double) * slot);
+  hipMemset(b->arena[1], 0, sizeof(double) * slot);
+  hipMemset(b->du, 0, du_bytes);
+  // operators per order
+  for (int i = 0; i < count; ++i) {
+    const int n = b->solves[i].n;
+    if (b->ops.count(n)) continue;
+    dg3d::LineOperator op;
+    if (!dg3d::make_line_operator(n, descs[i].mass, descs[i].stiff, &op))
+      return bail(DG3D_E_INVALID, "batch: singular mass matrix for order " + std::to_string(n - 1));
+    OrderOps& oo = b->ops[n];
+    const std::vector<double>* srcs[3] = {&op.A, &op.L0, &op.LN};
+    for (int k = 0; k < 3; ++k) {
+      const size_t cnt = srcs[k]->size();
+      std::vector<float> f(srcs[k]->begin(), srcs[k]->end());
+      if (bmalloc(&oo.opd[k], cnt) != hipSuccess || bmalloc(&oo.opf[k], cnt) != hipSuccess)
+        return bail(DG3D_E_NOMEM, "batch: operator allocation failed");
+      hipMemcpy(oo.opd[k], srcs[k]->data(), sizeof(double) * cnt, hipMemcpyHostToDevice);
+      hipMemcpy(oo.opf[k], f.data(), sizeof(float) * cnt, hipMemcpyHostToDevice);
+    }
+  }
+  // per-solve device data and descriptors
+  std::vector<dg3d::BatchSolve> ds(count);
+  std::vector<double> stage;
+  for (int i = 0; i < count; ++i) {
+    const dg3d_solve_desc& d = descs[i];
+    HostSolve& h = b->solves[i];
+    dg3d::BatchSolve& s = ds[i];
+    std::memset(&s, 0, sizeof(s));
+    dg3d::MeshArgs& m = s.mesh;
+    for (int a = 0; a < 3; ++a) {
+      m.nE[a] = h.nE[a];
+      m.periodic[a] = h.periodic[a];
+      std::vector<double> iw(h.nE[a]);
+      for (int k = 0; k < h.nE[a]; ++k) {
+        if (!(d.widths[a][k] > 0.0) || !std::isfinite(d.widths[a][k]))
+          return bail(DG3D_E_INVALID, "batch solve " + std::to_string(i) + ": element widths must be positive");
+        iw[k] = 1.0 / d.widths[a][k];
+      }
+      double* p = nullptr;
+      if (bmalloc(&p, iw.size()) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: geometry allocation failed");
+      b->allocs.push_back(p);
+      hipMemcpy(p, iw.data(), sizeof(double) * iw.size(), hipMemcpyHostToDevice);
+      m.inv_w[a] = p;
+    }
+    m.E = h.E;
+    m.fs = h.fs;
+    m.es = h.es;
+    m.stride[0] = (int64_t)h.nE[1] * h.nE[2];
+    m.stride[1] = h.nE[2];
+    m.stride[2] = 1;
+    m.div12 = dg3d::make_div32((uint32_t)(h.nE[1] * h.nE[2]));
+    m.div2 = dg3d::make_div32((uint32_t)h.nE[2]);
+    std::vector<double> rec((size_t)h.E * dg3d::kMatStride, 0.0);
+    for (int64_t k = 0; k < h.E; ++k) {
+      double* r = &rec[(size_t)k * dg3d::kMatStride];
+      if (!(d.c0[k] > 0.0) || !(d.rho0[k] > 0.0))
+        return bail(DG3D_E_INVALID, "batch solve " + std::to_string(i) + ": c0 and rho0 must be positive");
+      r[dg3d::MAT_C] = d.c0[k];
+      r[dg3d::MAT_RHO0] = d.rho0[k];
+      r[dg3d::MAT_BA] = d.b_over_a ? d.b_over_a[k] : 0.0;
+      r[dg3d::MAT_SIGMA] = d.sigma ? d.sigma[k] : 0.0;
+      r[dg3d::MAT_IRHO0] = 1.0 / d.rho0[k];
+    }
+    double* mat = nullptr;
+    if (bmalloc(&mat, rec.size()) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: material allocation failed");
+    b->allocs.push_back(mat);
+    hipMemcpy(mat, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice);
+    m.mat = mat;
+    for (int f = 0; f < 6; ++f) {
+      s.trace[f] = nullptr;
+      if (!d.halo[f] || h.periodic[2 - f / 2]) continue;
+      const int axis = 2 - f / 2;
+      const int64_t fe = (int64_t)h.nE[axis == 0 ? 1 : 0] * h.nE[axis == 2 ? 1 : 2];
+      const size_t cnt = (size_t)fe * h.n * h.n * 3;
+      double* p = nullptr;
+      if (bmalloc(&p, cnt) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: halo allocation failed");
+      b->allocs.push_back(p);
+      hipMemcpy(p, d.halo[f], sizeof(double) * cnt, hipMemcpyHostToDevice);
+      s.trace[f] = p;
+      h.has_halo[f] = true;
+    }
+    const OrderOps& oo = b->ops[h.n];
+    for (int k = 0; k < 3; ++k) {
+      s.opd[k] = oo.opd[k];
+      s.opf[k] = oo.opf[k];
+    }
+    s.slot0 = h.slot0;
+    s.dt = h.dt;
+    s.n = h.n;
+    // initial state: host [4][E][np] -> padded slots
+    stage.assign((size_t)4 * h.fs, 0.0);
+    for (int f = 0; f < 4; ++f)
+      for (int64_t el = 0; el < h.E; ++el)
+        std::memcpy(&stage[(size_t)f * h.fs + (size_t)el * h.es], d.state + ((size_t)f * h.E + el) * h.np,
+                    sizeof(double) * h.np);
+    e = hipMemcpy(b->arena[0] + h.slot0, stage.data(), sizeof(double) * stage.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(DG3D_E_HIP, std::string("batch: state upload: ") + hipGetErrorString(e));
+  }
+  if (bmalloc(&b->d_solves, ds.size()) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: descriptor allocation failed");
+  hipMemcpy(b->d_solves, ds.data(), sizeof(dg3d::BatchSolve) * ds.size(), hipMemcpyHostToDevice);
+  // one group per order present, largest first; task lists for both shapes
+  std::vector<int> orders;
+  for (const HostSolve& h : b->solves)
+    if (std::find(orders.begin(), orders.end(), h.n) == orders.end()) orders.push_back(h.n);
+  std::sort(orders.rbegin(), orders.rend());
+  for (int n : orders) {
+    dg3d_batch::Group g;
+    g.n = n;
+    for (int i = 0; i < count; ++i)
+      if (b->solves[i].n == n) g.nodes += b->solves[i].E * b->solves[i].np;
+    for (int v = 0; v < dg3d::kNumBatchVariants; ++v) {
+      const int epb = dg3d::batch_epb(v, n);
+      std::vector<dg3d::BatchTask> tasks;
+      for (int i = 0; i < count; ++i) {
+        if (b->solves[i].n != n) continue;
+        for (int64_t e0 = 0; e0 < b->solves[i].E; e0 += epb) tasks.push_back(dg3d::BatchTask{i, (int)e0});
+      }
+      if (tasks.size() >= (size_t)INT32_MAX) return bail(DG3D_E_INVALID, "batch: too many element groups");
+      if (bmalloc(&g.d_tasks[v], tasks.size()) != hipSuccess) {
+        b->groups.push_back(g);
+        return bail(DG3D_E_NOMEM, "batch: task list allocation failed");
+      }
+      hipMemcpy(g.d_tasks[v], tasks.data(), sizeof(dg3d::BatchTask) * tasks.size(), hipMemcpyHostToDevice);
+      g.ntasks[v] = (int)tasks.size();
+      const int resident = dg3d::batch_max_grid(n, v, precision == DG3D_MIXED ? 1 : 0);
+      g.grid[v] = std::max(1, std::min(g.ntasks[v], resident > 0 ? resident : 1024));
+    }
+    g.variant = 0;
+    e = hipStreamCreate(&g.stream);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g.done, hipEventDisableTiming);
+    b->groups.push_back(g);
+    if (e != hipSuccess) return bail(DG3D_E_HIP, std::string("batch streams: ") + hipGetErrorString(e));
+  }
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) return bail(DG3D_E_HIP, std::string("batch setup: ") + hipGetErrorString(e));
+  *out = b;
+  return DG3D_OK;
+}
+
+int dg3d_batch_destroy(dg3d_batch* b) {
+  free_batch(b);
+  return DG3D_OK;
+}
+
+int dg3d_batch_set_variant(dg3d_batch* b, int variant) {
+  if (!b) return DG3D_E_INVALID;
+  if (variant < 0 || variant >= dg3d::kNumBatchVariants) return b->fail(DG3D_E_INVALID, "batch: variant out of range");
+  for (auto& g : b->groups) g.variant = variant;
+  b->tuned = true;
+  drop_graphs(b);
+  return DG3D_OK;
+}
+
+int dg3d_batch_autotune(dg3d_batch* b, int* variants) {
+  if (!b) return DG3D_E_INVALID;
+  const int rc = batch_autotune(b);
+  if (rc) return rc;
+  drop_graphs(b);
+  if (variants)
+    for (size_t i = 0; i < b->groups.size(); ++i) variants[i] = b->groups[i].variant;
+  return DG3D_OK;
+}
+
+int dg3d_batch_step(dg3d_batch* b, int nsteps) {
+  if (!b) return DG3D_E_INVALID;
+  if (nsteps < 0) return b->fail(DG3D_E_INVALID, "batch step: nsteps < 0");
+  if (nsteps == 0) return DG3D_OK;
+  if (!b->tuned) {
+    const int rc = batch_autotune(b);
+    if (rc) return rc;
+  }
+  hipError_t e = hipSuccess;
+  if (!b->graph[0] || !b->graph[1]) {
+    e = build_graphs(b);
+    if (e != hipSuccess) return b->hip_fail(e, "batch graph capture");
+  }
+  e = hipEventRecord(b->ev0, b->stream);
+  int s = 0;
+  if (e == hipSuccess && b->halos_pending) {
+    // the run's first stage reads the initial halo traces: launched directly, not from a graph
+    e = launch_step(b, b->cur, true);
+    b->cur = 1 - b->cur;
+    b->halos_pending = false;
+    s = 1;
+  }
+  for (; s < nsteps && e == hipSuccess; ++s) {
+    e = hipGraphLaunch(b->graph[b->cur], b->stream);
+    b->cur = 1 - b->cur;
+  }
+  if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
+  if (e != hipSuccess) return b->hip_fail(e, "batch step");
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, b->ev0, b->ev1);
+  b->last_ms = ms;
+  b->last_launches = 5 * nsteps;
+  return DG3D_OK;
+}
+
+int dg3d_batch_get_state(dg3d_batch* b, int solve, double* host) {
+  if (!b || !host) return DG3D_E_INVALID;
+  if (solve < 0 || solve >= (int)b->solves.size()) return b->fail(DG3D_E_INVALID, "batch: solve out of range");
+  const HostSolve& h = b->solves[solve];
+  hipError_t e = hipStreamSynchronize(b->stream);
+  std::vector<double> stage((size_t)4 * h.fs);
+  if (e == hipSuccess)
+    e = hipMemcpy(stage.data(), b->arena[b->cur] + h.slot0, sizeof(double) * stage.size(), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return b->hip_fail(e, "batch get_state");
+  for (int f = 0; f < 4; ++f)
+    for (int64_t el = 0; el < h.E; ++el)
+      std::memcpy(host + ((size_t)f * h.E + el) * h.np, &stage[(size_t)f * h.fs + (size_t)el * h.es],
+                  sizeof(double) * h.np);
+  return DG3D_OK;
+}
+
+int dg3d_batch_info(const dg3d_batch* b, int64_t* total_nodes, int* num_orders, int* orders, int* variants,
+                    int* tasks, int* grids) {
+  if (!b) return DG3D_E_INVALID;
+  int64_t nodes = 0;
+  for (const HostSolve& h : b->solves) nodes += h.E * h.np;
+  if (total_nodes) *total_nodes = nodes;
+  if (num_orders) *num_orders = (int)b->groups.size();
+  for (size_t i = 0; i < b->groups.size(); ++i) {
+    const auto& g = b->groups[i];
+    const int v = g.variant < 0 ? 0 : g.variant;
+    if (orders) orders[i] = g.n - 1;
+    if (variants) variants[i] = g.variant;
+    if (tasks) tasks[i] = g.ntasks[v];
+    if (grids) grids[i] = g.grid[v];
+  }
+  return DG3D_OK;
+}
+
+int dg3d_batch_last_step_timing(const dg3d_batch* b, double* avg_stage_ms, int* launches) {
+  if (!b) return DG3D_E_INVALID;
+  if (avg_stage_ms) *avg_stage_ms = b->last_launches ? b->last_ms / b->last_launches : 0.0;
+  if (launches) *launches = b->last_launches;
+  return DG3D_OK;
+}
+
+}  // extern "C"

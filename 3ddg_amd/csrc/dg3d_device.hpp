@@ -8,6 +8,16 @@ namespace dg3d {
 
 constexpr int kMaxN = 8;  // n = N+1 <= 8 (orders 1..7)
 
+// LSRK(5,4) of Carpenter & Kennedy (NASA TM-109112, 1994), 2N-storage: per stage
+// du = A_s du + dt rhs(t + C_s dt); q = q + B_s du.
+inline constexpr double kLsA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                                   -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+inline constexpr double kLsB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                                   1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                                   2277821191437.0 / 14882151754819.0};
+inline constexpr double kLsC[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                                   2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
+
 // Term mask bits of the RHS kernel: UFromV(axis, mode) = bit (2*axis + mode), sponge = bit 6.
 constexpr unsigned kTermSponge = 1u << 6;
 constexpr unsigned kTermAll = 0x7Fu;
@@ -99,6 +109,41 @@ struct RhsLaunch {
 hipError_t launch_rhs(const RhsLaunch& L, hipStream_t s);
 constexpr int kNumVariants = 4;
 constexpr int kNumOrders = 4;
+
+// Batched stages: many independent blocks (solves) of different sizes and orders advanced by one
+// persistent launch per LSRK54 stage. Each solve owns a region of 4 * mesh.fs slots in the
+// state arenas, starting at slot0; its element groups are tasks that the workgroups take in
+// turn.
+struct BatchSolve {
+  MeshArgs mesh;             // the solve's block (its own inv_w and material arrays)
+  int64_t slot0;             // first slot of the solve's region in the arenas
+  const double* trace[6];    // halo traces for the first stage of a run, or null
+  const double* opd[3];      // line operators of the solve's order
+  const float* opf[3];
+  double dt;
+  int n;
+};
+struct BatchTask {
+  int solve;
+  int e0;  // first element of the group
+};
+struct BatchLaunch {
+  const BatchSolve* solves;
+  const BatchTask* tasks;
+  int ntasks;
+  int mixed;         // 0: FP64 arithmetic and du; 1: FP32 arithmetic and du
+  int use_halo;      // 1 on the first stage of a run
+  const double* q_in;
+  double* q_out;
+  void* du;
+  double a, b;       // LSRK54 stage coefficients
+};
+// Per order, two elements-per-workgroup shapes (the stage kernel's EPB0 and EPB1), autotuned.
+constexpr int kNumBatchVariants = 2;
+int batch_epb(int variant, int n);
+// One persistent launch over the tasks of the solves of order n-1 (B.tasks holds only those).
+hipError_t launch_rhs_batch(const BatchLaunch& B, int n, int variant, int grid, hipStream_t s);
+int batch_max_grid(int n, int variant, int mixed);  // resident workgroups on the device
 
 // Exterior trace of the source face at stage time x.t (halo trace + boundary source) into `out`
 // ([faceElem][n*n][3]); the RHS kernel then reads it as that face's halo.

@@ -375,30 +375,40 @@ __device__ __forceinline__ int face_neighbour(const MeshArgs& m, int he, const i
   return (pos > 0) ? he - st : (per ? he + (ne - 1) * st : -1 - fe);
 }
 
-template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT, int CONTRACT = 0>
-__global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunch L) {
+// LDS of one stage workgroup: EPB elements of order NN-1.
+template <int NN, int EPB, typename CT, int CONTRACT>
+struct StageSmem {
+  static constexpr int NP = NN * NN * NN, NF2 = NN * NN;
+  CT sF[EPB * 4 * NP];                    // P/rho0, m_x, m_y, m_z per node
+  CT sFl[EPB * 6 * NF2 * 2];              // scaled flux pair per face node
+  CT sK[EPB * 6 * kNK];                   // flux coefficients per face
+  CT sIw[EPB * 3];                        // 1/width per element and axis
+  float sOut[CONTRACT ? EPB * 6 * NP : 1];  // matrix-core contraction results
+  CT sA[NN * NN];
+  CT sL0[NN];
+  CT sLN[NN];
+};
+
+// One workgroup's share of a stage: elements e0 .. e0+EPB-1 of the block described by m (field
+// arrays in L; traces: the 6 exterior face traces in memory, or null for none). NT is the
+// workgroup size (>= EPB * ES); every lane must call it, since it holds barriers.
+template <int NN, int EPB, int NT, typename CT, int EPI, typename DUT, int CONTRACT>
+__device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m, const double* const* traces,
+                                           const int e0, const int tid, StageSmem<NN, EPB, CT, CONTRACT>& sm) {
   using S = Geo<NN, EPB>;
-  constexpr int NP = S::NP, NF2 = S::NF2, NT = S::NT;
+  constexpr int NP = S::NP, NF2 = S::NF2;
+  static_assert(NT >= EPB * S::ES, "workgroup too small for the element group");
   constexpr int NITEMS = EPB * 6 * NF2;            // face work items of the workgroup
   constexpr int R = (NITEMS + NT - 1) / NT;        // face items per lane
-  __shared__ CT sF[EPB * 4 * NP];                  // P/rho0, m_x, m_y, m_z per node
-  __shared__ CT sFl[EPB * 6 * NF2 * 2];            // scaled flux pair per face node
-  __shared__ CT sK[EPB * 6 * kNK];                 // flux coefficients per face
-  __shared__ CT sIw[EPB * 3];                      // 1/width per element and axis
-  __shared__ float sOut[CONTRACT ? EPB * 6 * NP : 1];  // matrix-core contraction results
-  __shared__ CT sA[NN * NN];
-  __shared__ CT sL0[NN];
-  __shared__ CT sLN[NN];
-
-  const MeshArgs& m = L.mesh;
+  CT* const sF = sm.sF;
+  CT* const sFl = sm.sFl;
+  CT* const sK = sm.sK;
+  CT* const sIw = sm.sIw;
+  float* const sOut = sm.sOut;
+  CT* const sA = sm.sA;
+  CT* const sL0 = sm.sL0;
+  CT* const sLN = sm.sLN;
   const unsigned mask = L.mask;
-  const int tid = threadIdx.x;
-  int64_t grp;
-  if (L.xcd_remap == 3)
-    grp = tiled_group(remap_block(blockIdx.x, gridDim.x, 1, 1), m.nE[0], m.nE[1], m.nE[2], EPB);
-  else
-    grp = remap_block(blockIdx.x, gridDim.x, L.xcd_remap, order_run(m.nE[2], EPB));
-  const int e0 = (int)(grp * EPB);
   const int E = (int)m.E;
   const int64_t fs = m.fs;
 
@@ -474,12 +484,7 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
           else if (axis == 1) nu[r] = bload<double>(rq[2], oi);
           else nu[r] = bload<double>(rq[3], oi);
         } else {
-          const double* tr = L.ext.trace[0];
-          if (dir == 1) tr = L.ext.trace[1];
-          if (dir == 2) tr = L.ext.trace[2];
-          if (dir == 3) tr = L.ext.trace[3];
-          if (dir == 4) tr = L.ext.trace[4];
-          if (dir == 5) tr = L.ext.trace[5];
+          const double* tr = traces ? traces[dir] : nullptr;
           if (tr) {
             const int64_t ti = ((int64_t)(-1 - nb) * NF2 + fn) * 3;
             nrho[r] = tr[ti];
@@ -660,6 +665,79 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
       if (L.a != 0.0) v += pre[f] * L.a;
       bstore<double>(v, make_rsrc(L.out + f * fs, fb64), gi);
     }
+  }
+}
+
+template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT, int CONTRACT = 0>
+__global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunch L) {
+  __shared__ StageSmem<NN, EPB, CT, CONTRACT> sm;
+  const MeshArgs& m = L.mesh;
+  int64_t grp;
+  if (L.xcd_remap == 3)
+    grp = tiled_group(remap_block(blockIdx.x, gridDim.x, 1, 1), m.nE[0], m.nE[1], m.nE[2], EPB);
+  else
+    grp = remap_block(blockIdx.x, gridDim.x, L.xcd_remap, order_run(m.nE[2], EPB));
+  stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI, DUT, CONTRACT>(L, m, L.ext.trace, (int)(grp * EPB),
+                                                                (int)threadIdx.x, sm);
+}
+
+// ---- batched persistent stage kernels ---------------------------------------------------
+// One kernel per order: a persistent loop over the element groups (tasks) of every solve of
+// that order in the batch. Workgroup w takes tasks w, w + grid, ... (a static schedule: every
+// workgroup leaves the loop after the last task). The descriptors do not change during the
+// launch and are read through the constant address space (scalar loads the compiler may repeat
+// instead of holding them in registers).
+using ConstSolvePtr = const __attribute__((address_space(4))) BatchSolve*;
+using ConstTaskPtr = const __attribute__((address_space(4))) BatchTask*;
+
+__device__ __forceinline__ MeshArgs load_mesh(ConstSolvePtr sd) {
+  MeshArgs m;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    m.nE[a] = sd->mesh.nE[a];
+    m.periodic[a] = sd->mesh.periodic[a];
+    m.stride[a] = sd->mesh.stride[a];
+    m.inv_w[a] = sd->mesh.inv_w[a];
+  }
+  m.E = sd->mesh.E;
+  m.fs = sd->mesh.fs;
+  m.es = sd->mesh.es;
+  m.div12.m = sd->mesh.div12.m;
+  m.div12.s = sd->mesh.div12.s;
+  m.div2.m = sd->mesh.div2.m;
+  m.div2.s = sd->mesh.div2.s;
+  m.mat = sd->mesh.mat;
+  return m;
+}
+
+template <int NN, int EPB, typename CT, typename DUT>
+__global__ void __launch_bounds__((Geo<NN, EPB>::NT)) k_rhs_batch(const BatchLaunch B) {
+  __shared__ StageSmem<NN, EPB, CT, 0> sm;
+  const ConstTaskPtr tasks = (ConstTaskPtr)B.tasks;
+  for (int t = blockIdx.x; t < B.ntasks; t += gridDim.x) {
+    __syncthreads();  // the previous task's phase 3 still reads LDS
+    const int solve = tasks[t].solve, e0 = tasks[t].e0;
+    const ConstSolvePtr sd = (ConstSolvePtr)(B.solves + solve);
+    RhsLaunch L{};
+    L.n = NN;
+    L.mixed = B.mixed;
+    L.epilogue = EPI_LSRK;
+    L.mask = kTermAll;
+    L.ext.src_face = -1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      L.opd[i] = sd->opd[i];
+      L.opf[i] = sd->opf[i];
+    }
+    L.q_in = B.q_in + sd->slot0;
+    L.q_out = B.q_out + sd->slot0;
+    L.du = static_cast<DUT*>(B.du) + sd->slot0;
+    L.a = B.a;
+    L.b = B.b;
+    L.dt = sd->dt;
+    const MeshArgs m = load_mesh(sd);
+    const double* const* traces = B.use_halo ? (const double* const*)(B.solves + solve)->trace : nullptr;
+    stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI_LSRK, DUT, 0>(L, m, traces, e0, (int)threadIdx.x, sm);
   }
 }
 
@@ -987,6 +1065,77 @@ hipError_t launch_cfl(const MeshArgs& m, double dxi_half, double* part, hipStrea
   hipLaunchKernelGGL(k_cfl, dim3(reduce_blocks(m.E)), dim3(kRedThreads), 0, s, m, dxi_half, part);
   return hipGetLastError();
 }
+int batch_epb(int variant, int n) {
+  switch (n) {
+    case 2: return variant ? Variants<2>::EPB1 : Variants<2>::EPB0;
+    case 3: return variant ? Variants<3>::EPB1 : Variants<3>::EPB0;
+    case 4: return variant ? Variants<4>::EPB1 : Variants<4>::EPB0;
+    case 5: return variant ? Variants<5>::EPB1 : Variants<5>::EPB0;
+    case 6: return variant ? Variants<6>::EPB1 : Variants<6>::EPB0;
+    case 7: return variant ? Variants<7>::EPB1 : Variants<7>::EPB0;
+    case 8: return variant ? Variants<8>::EPB1 : Variants<8>::EPB0;
+    default: return 0;
+  }
+}
+
+template <int NN, int EPB>
+static hipError_t launch_batch_shape(const BatchLaunch& B, int grid, hipStream_t s) {
+  constexpr int NT = Geo<NN, EPB>::NT;
+  if (B.mixed) hipLaunchKernelGGL((k_rhs_batch<NN, EPB, float, float>), dim3(grid), dim3(NT), 0, s, B);
+  else hipLaunchKernelGGL((k_rhs_batch<NN, EPB, double, double>), dim3(grid), dim3(NT), 0, s, B);
+  return hipGetLastError();
+}
+template <int NN>
+static hipError_t launch_batch_n(const BatchLaunch& B, int variant, int grid, hipStream_t s) {
+  if (variant == 0) return launch_batch_shape<NN, Variants<NN>::EPB0>(B, grid, s);
+  return launch_batch_shape<NN, Variants<NN>::EPB1>(B, grid, s);
+}
+
+hipError_t launch_rhs_batch(const BatchLaunch& B, int n, int variant, int grid, hipStream_t s) {
+  if (B.ntasks <= 0) return hipSuccess;
+  if (grid < 1) grid = 1;
+  if (variant < 0 || variant >= kNumBatchVariants) return hipErrorInvalidValue;
+  switch (n) {
+    case 2: return launch_batch_n<2>(B, variant, grid, s);
+    case 3: return launch_batch_n<3>(B, variant, grid, s);
+    case 4: return launch_batch_n<4>(B, variant, grid, s);
+    case 5: return launch_batch_n<5>(B, variant, grid, s);
+    case 6: return launch_batch_n<6>(B, variant, grid, s);
+    case 7: return launch_batch_n<7>(B, variant, grid, s);
+    case 8: return launch_batch_n<8>(B, variant, grid, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int NN, int EPB>
+static int resident_shape(int mixed) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  constexpr int NT = Geo<NN, EPB>::NT;
+  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, float, float>, NT, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, double, double>, NT, 0);
+  if (e != hipSuccess || per < 1) per = 1;
+  return cus * per;
+}
+template <int NN>
+static int resident_n(int variant, int mixed) {
+  return variant == 0 ? resident_shape<NN, Variants<NN>::EPB0>(mixed) : resident_shape<NN, Variants<NN>::EPB1>(mixed);
+}
+
+int batch_max_grid(int n, int variant, int mixed) {
+  switch (n) {
+    case 2: return resident_n<2>(variant, mixed);
+    case 3: return resident_n<3>(variant, mixed);
+    case 4: return resident_n<4>(variant, mixed);
+    case 5: return resident_n<5>(variant, mixed);
+    case 6: return resident_n<6>(variant, mixed);
+    case 7: return resident_n<7>(variant, mixed);
+    case 8: return resident_n<8>(variant, mixed);
+    default: return 0;
+  }
+}
+
 hipError_t launch_probe(const double* reg, const MeshArgs& m, int n, const int64_t* nodes, int count, double* out,
                         hipStream_t s) {
   if (count <= 0) return hipSuccess;

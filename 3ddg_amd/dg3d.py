@@ -52,6 +52,24 @@ class Source(ctypes.Structure):
     ]
 
 
+class SolveDesc(ctypes.Structure):
+    _fields_ = [
+        ("order", ctypes.c_int),
+        ("num_elements", ctypes.c_int * 3),
+        ("periodic", ctypes.c_int * 3),
+        ("mass", _dp),
+        ("stiff", _dp),
+        ("widths", _dp * 3),
+        ("c0", _dp),
+        ("rho0", _dp),
+        ("b_over_a", _dp),
+        ("sigma", _dp),
+        ("state", _dp),
+        ("halo", _dp * 6),
+        ("dt", ctypes.c_double),
+    ]
+
+
 # every entry point declared in include/dg3d.h, with its ctypes signature
 _SIGS = {
     "dg3d_abi_version": (ctypes.c_int, []),
@@ -82,6 +100,17 @@ _SIGS = {
     "dg3d_get_probes": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip]),
     "dg3d_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "dg3d_last_step_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip]),
+    "dg3d_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(SolveDesc), ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+    "dg3d_batch_create_error": (ctypes.c_char_p, []),
+    "dg3d_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "dg3d_batch_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "dg3d_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, _ip]),
+    "dg3d_batch_set_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dg3d_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dg3d_batch_get_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
+    "dg3d_batch_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _ip, _ip, _ip, _ip, _ip]),
+    "dg3d_batch_last_step_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip]),
     "dg3d_basis_generate": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]),
     "dg3d_basis_write_files": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "dg3d_basis_read_files": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _dp, _dp, _dp, _dp]),
@@ -315,4 +344,97 @@ class Context:
         ms = ctypes.c_double()
         n = ctypes.c_int()
         self._check(self._L.dg3d_last_step_timing(self._h, ctypes.byref(ms), ctypes.byref(n)), "timing")
+        return ms.value, n.value
+
+
+# ---- batched solves -----------------------------------------------------------
+
+class Batch:
+    """Independent blocks of different sizes and orders advanced together by LSRK54 (the batched
+    entry points of include/dg3d.h). `solves` is a sequence of dicts with the keys order,
+    num_elements, mass, stiff, widths (3 arrays), c0, rho0, b_over_a, sigma, state (4 fields,
+    dg3d_set_state layout), dt, and optionally periodic and halo (6 traces or None)."""
+
+    def __init__(self, solves, precision=MIXED):
+        self._L = lib()
+        self._keep = []
+        descs = (SolveDesc * len(solves))()
+        self.sizes = []
+        for d, s in zip(descs, solves):
+            order = int(s["order"])
+            d.order = order
+            d.num_elements[:] = [int(x) for x in s["num_elements"]]
+            d.periodic[:] = [int(x) for x in s.get("periodic", (0, 0, 0))]
+            d.mass = self._arr(s["mass"])
+            d.stiff = self._arr(s["stiff"])
+            for a in range(3):
+                d.widths[a] = self._arr(s["widths"][a])
+            for name in ("c0", "rho0", "b_over_a", "sigma", "state"):
+                setattr(d, name, self._arr(s.get(name)))
+            halo = s.get("halo") or [None] * 6
+            for f in range(6):
+                d.halo[f] = self._arr(halo[f])
+            d.dt = float(s["dt"])
+            E = int(np.prod(s["num_elements"]))
+            self.sizes.append((E, (order + 1) ** 3))
+        h = ctypes.c_void_p()
+        rc = self._L.dg3d_batch_create(len(solves), descs, int(precision), ctypes.byref(h))
+        if rc:
+            raise Dg3dError(rc, self._L.dg3d_batch_create_error().decode())
+        self._h = h
+        self._keep = []
+
+    def _arr(self, a):
+        if a is None:
+            return None
+        a = _f64(a).ravel()
+        self._keep.append(a)
+        return _ptr(a)
+
+    def _check(self, rc, what):
+        if rc:
+            raise Dg3dError(rc, f"{what}: {self._L.dg3d_batch_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.dg3d_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self):
+        out = (ctypes.c_int * 8)()
+        self._check(self._L.dg3d_batch_autotune(self._h, out), "batch_autotune")
+        return list(out[: self.info()["num_orders"]])
+
+    def set_variant(self, variant):
+        self._check(self._L.dg3d_batch_set_variant(self._h, int(variant)), "batch_set_variant")
+
+    def step(self, nsteps):
+        self._check(self._L.dg3d_batch_step(self._h, int(nsteps)), "batch_step")
+
+    def get_state(self, solve):
+        E, np_ = self.sizes[solve]
+        out = np.zeros(4 * E * np_)
+        self._check(self._L.dg3d_batch_get_state(self._h, int(solve), _ptr(out)), "batch_get_state")
+        return out
+
+    def info(self):
+        nodes = ctypes.c_int64()
+        k = ctypes.c_int()
+        orders, variants, tasks, grids = [(ctypes.c_int * 8)() for _ in range(4)]
+        self._check(self._L.dg3d_batch_info(self._h, ctypes.byref(nodes), ctypes.byref(k), orders, variants, tasks,
+                                            grids), "batch_info")
+        n = k.value
+        return dict(total_nodes=nodes.value, num_orders=n, orders=list(orders[:n]), variants=list(variants[:n]),
+                    tasks=list(tasks[:n]), grids=list(grids[:n]))
+
+    def last_step_timing(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int()
+        self._check(self._L.dg3d_batch_last_step_timing(self._h, ctypes.byref(ms), ctypes.byref(n)), "batch_timing")
         return ms.value, n.value

@@ -324,3 +324,47 @@ def cfl_dt_host(p):
     inv = (1.0 / p.widths[0][:, None, None] + 1.0 / p.widths[1][None, :, None]
            + 1.0 / p.widths[2][None, None, :]).reshape(-1)
     return p.cfl * float(np.min(0.5 * dxi / (inv * p.c0)))
+
+
+def config_c4(num_solves=256, steps=100, seed=0):
+    """Batch of independent unit-cube solves. Per solve, from mt19937_64(seed) in this order:
+    elements per side uniform in 4..16, order N uniform in 2..6, pulse amplitude uniform in
+    [1e4, 1e6] Pa, beta uniform in [0, 5] with B/A = max(0, 2 (beta - 1)). Each: water (c=1500,
+    rho0=1000), Gaussian pressure pulse of width 0.08 at the centre, 2-element quadratic sponge
+    (sigma_max = 3c/(2h)), LSRK54 at CFL 0.3."""
+    rng = MT19937_64(seed)
+    out = []
+    for s in range(num_solves):
+        ne = 4 + rng() % 13
+        order = 2 + rng() % 5
+        amp = 1.0e4 + (1.0e6 - 1.0e4) * rng.uniform01()
+        beta = 5.0 * rng.uniform01()
+        ba = max(0.0, 2.0 * (beta - 1.0))
+        nE = (ne, ne, ne)
+        h = 1.0 / ne
+        p = _base(f"c4_{s}", order, nE, (0, 0, 0), _uniform_widths(nE, (h, h, h)), 1500.0, 1000.0, ba)
+        p.ic = dict(gauss=(1, 1, 1), amp_rho=amp / 1500.0 ** 2, amp_vel=(0.0, 0.0, 0.0),
+                    sigma=1.0 / (2 * 0.08 ** 2), center=(0.5, 0.5, 0.5))
+        set_sponge(p, (2, 2, 2, 2, 2, 2), 3 * 1500.0 / (2 * h), ramp=1)
+        p.scheme, p.cfl, p.steps = dg3d.LSRK54, 0.3, steps
+        out.append(p)
+    return out
+
+
+def batch_solve(p):
+    """The dg3d_batch_create description of one problem: the arrays make_context would upload,
+    the initial halo traces, and the CFL time step."""
+    halos = halo_traces(p)
+    return dict(order=p.order, num_elements=p.num_elements, periodic=p.periodic, mass=p.mass, stiff=p.stiff,
+                widths=p.widths, c0=p.c0, rho0=p.rho0, b_over_a=p.b_over_a, sigma=p.sigma,
+                state=initial_state(p),
+                halo=[None if (tr is None or p.periodic[FACE_AXIS[f]]) else tr for f, tr in enumerate(halos)],
+                dt=cfl_dt_host(p))
+
+
+def make_batch(problems, precision=dg3d.MIXED):
+    """A device batch of the problems (no boundary sources)."""
+    for p in problems:
+        if p.source.face >= 0:
+            raise ValueError(f"{p.name}: the batched path has no boundary source")
+    return dg3d.Batch([batch_solve(p) for p in problems], precision=precision)

@@ -191,6 +191,51 @@ int dg3d_sync(dg3d_ctx* ctx);
  * context stream), and the number of stage launches it averaged. */
 int dg3d_last_step_timing(dg3d_ctx* ctx, double* avg_stage_ms, int* launches);
 
+/* ---- batched solves (no counterpart in the reference) -------------------------------------
+ * Many independent blocks of different element counts and orders, each what one dg3d_ctx
+ * would hold (the reference builds one such block per MPI rank, Grid.cpp:176-375), advanced
+ * together by LSRK54: per stage, one persistent launch per order present covers all solves of
+ * that order (the launches run as parallel graph branches), whole steps are replayed from HIP
+ * graphs, and elements per workgroup are autotuned per order. The first stage of the first dg3d_batch_step reads the halo traces given
+ * here (as dg3d_set_halo on register 0 before dg3d_step); later stages see zero halos. No
+ * boundary source. Each solve follows the trajectory of dg3d_step on a context of its own with
+ * the same precision (same per-node arithmetic). */
+typedef struct dg3d_batch dg3d_batch;
+typedef struct dg3d_solve_desc {
+  int order;                 /* 1..7 */
+  int num_elements[3];
+  int periodic[3];
+  const double* mass;        /* (order+1)^2, as dg3d_desc */
+  const double* stiff;
+  const double* widths[3];   /* element widths per axis (num_elements[a] each) */
+  const double* c0;          /* per element, E = product of num_elements */
+  const double* rho0;
+  const double* b_over_a;    /* NULL = 0 */
+  const double* sigma;       /* NULL = 0 */
+  const double* state;       /* initial rho', u, v, w: [4][E][(order+1)^3], as dg3d_set_state */
+  const double* halo[6];     /* first-stage halo traces per face (as dg3d_set_halo), or NULL */
+  double dt;                 /* time step of this solve */
+} dg3d_solve_desc;
+
+int dg3d_batch_create(int count, const dg3d_solve_desc* solves, int precision, dg3d_batch** out);
+const char* dg3d_batch_create_error(void);
+int dg3d_batch_destroy(dg3d_batch* batch);
+const char* dg3d_batch_last_error(const dg3d_batch* batch);
+/* Shape per order: variant 0 or 1 (the stage kernel's two elements-per-workgroup shapes).
+ * dg3d_batch_autotune writes one variant per order, in dg3d_batch_info's order. */
+int dg3d_batch_autotune(dg3d_batch* batch, int* variants);
+int dg3d_batch_set_variant(dg3d_batch* batch, int variant);
+/* nsteps LSRK54 steps of every solve, each with its own dt; synchronises. */
+int dg3d_batch_step(dg3d_batch* batch, int nsteps);
+/* State of one solve in the dg3d_get_state layout (4 fields). */
+int dg3d_batch_get_state(dg3d_batch* batch, int solve, double* host);
+/* Nodes over all solves; per order present (descending, up to 7 entries per array): the order,
+ * its tuned variant (-1 before tuning), its element-group count and its persistent grid. */
+int dg3d_batch_info(const dg3d_batch* batch, int64_t* total_nodes, int* num_orders, int* orders, int* variants,
+                    int* tasks, int* grids);
+/* Average stage time of the last dg3d_batch_step (HIP events around the whole run). */
+int dg3d_batch_last_step_timing(const dg3d_batch* batch, double* avg_stage_ms, int* launches);
+
 /* ---- host-side setup helpers (no device work) ------------------------------ */
 
 /* LGL basis for order N generated in FP64 (replaces the missing hardcoded_coeff*.txt and
