@@ -60,9 +60,9 @@ struct dg3d_batch {
   // per order present (descending n): task lists, grids and the tuned shape
   struct Group {
     int n = 0;
-    dg3d::BatchTask* d_tasks[dg3d::kNumBatchVariants] = {nullptr, nullptr};
-    int ntasks[dg3d::kNumBatchVariants] = {0, 0};
-    int grid[dg3d::kNumBatchVariants] = {0, 0};
+    dg3d::BatchTask* d_tasks[dg3d::kNumBatchVariants] = {};
+    int ntasks[dg3d::kNumBatchVariants] = {};
+    int grid[dg3d::kNumBatchVariants] = {};
     int variant = -1;
     int64_t nodes = 0;
     hipStream_t stream = nullptr;      // branch of the stage's fork/join
@@ -186,7 +186,7 @@ hipError_t build_graphs(dg3d_batch* b) {
   return hipSuccess;
 }
 
-// Per order, times both elements-per-workgroup shapes on a later-stage launch (reads and writes
+// Per order, times every variant (shape x persistent or one task per workgroup) on a later-stage launch (reads and writes
 // du) that writes only arena 1 - cur and du; du is cleared first, and the run's next stage 0
 // overwrites both.
 int batch_autotune(dg3d_batch* b) {

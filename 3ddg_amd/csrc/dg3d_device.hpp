@@ -138,8 +138,9 @@ struct BatchLaunch {
   void* du;
   double a, b;       // LSRK54 stage coefficients
 };
-// Per order, two elements-per-workgroup shapes (the stage kernel's EPB0 and EPB1), autotuned.
-constexpr int kNumBatchVariants = 2;
+// Per order, autotuned: bit 0 picks the elements-per-workgroup shape (the stage kernel's EPB0 or
+// EPB1), bit 1 one task per workgroup instead of the persistent loop.
+constexpr int kNumBatchVariants = 4;
 int batch_epb(int variant, int n);
 // One persistent launch over the tasks of the solves of order n-1 (B.tasks holds only those).
 hipError_t launch_rhs_batch(const BatchLaunch& B, int n, int variant, int grid, hipStream_t s);

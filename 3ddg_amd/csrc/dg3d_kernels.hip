@@ -710,12 +710,14 @@ __device__ __forceinline__ MeshArgs load_mesh(ConstSolvePtr sd) {
   return m;
 }
 
-template <int NN, int EPB, typename CT, typename DUT>
+// PERSIST = false is the same kernel with one task per workgroup (grid = task count), for the
+// autotuner to weigh against the persistent form.
+template <int NN, int EPB, typename CT, typename DUT, bool PERSIST>
 __global__ void __launch_bounds__((Geo<NN, EPB>::NT)) k_rhs_batch(const BatchLaunch B) {
   __shared__ StageSmem<NN, EPB, CT, 0> sm;
   const ConstTaskPtr tasks = (ConstTaskPtr)B.tasks;
-  for (int t = blockIdx.x; t < B.ntasks; t += gridDim.x) {
-    __syncthreads();  // the previous task's phase 3 still reads LDS
+  for (int t = blockIdx.x; t < B.ntasks; t += (PERSIST ? gridDim.x : B.ntasks)) {
+    if (PERSIST) __syncthreads();  // the previous task's phase 3 still reads LDS
     const int solve = tasks[t].solve, e0 = tasks[t].e0;
     const ConstSolvePtr sd = (ConstSolvePtr)(B.solves + solve);
     RhsLaunch L{};
@@ -1066,29 +1068,36 @@ hipError_t launch_cfl(const MeshArgs& m, double dxi_half, double* part, hipStrea
   return hipGetLastError();
 }
 int batch_epb(int variant, int n) {
+  const bool alt = (variant & 1) != 0;
   switch (n) {
-    case 2: return variant ? Variants<2>::EPB1 : Variants<2>::EPB0;
-    case 3: return variant ? Variants<3>::EPB1 : Variants<3>::EPB0;
-    case 4: return variant ? Variants<4>::EPB1 : Variants<4>::EPB0;
-    case 5: return variant ? Variants<5>::EPB1 : Variants<5>::EPB0;
-    case 6: return variant ? Variants<6>::EPB1 : Variants<6>::EPB0;
-    case 7: return variant ? Variants<7>::EPB1 : Variants<7>::EPB0;
-    case 8: return variant ? Variants<8>::EPB1 : Variants<8>::EPB0;
+    case 2: return alt ? Variants<2>::EPB1 : Variants<2>::EPB0;
+    case 3: return alt ? Variants<3>::EPB1 : Variants<3>::EPB0;
+    case 4: return alt ? Variants<4>::EPB1 : Variants<4>::EPB0;
+    case 5: return alt ? Variants<5>::EPB1 : Variants<5>::EPB0;
+    case 6: return alt ? Variants<6>::EPB1 : Variants<6>::EPB0;
+    case 7: return alt ? Variants<7>::EPB1 : Variants<7>::EPB0;
+    case 8: return alt ? Variants<8>::EPB1 : Variants<8>::EPB0;
     default: return 0;
   }
 }
 
-template <int NN, int EPB>
+template <int NN, int EPB, bool PERSIST>
 static hipError_t launch_batch_shape(const BatchLaunch& B, int grid, hipStream_t s) {
   constexpr int NT = Geo<NN, EPB>::NT;
-  if (B.mixed) hipLaunchKernelGGL((k_rhs_batch<NN, EPB, float, float>), dim3(grid), dim3(NT), 0, s, B);
-  else hipLaunchKernelGGL((k_rhs_batch<NN, EPB, double, double>), dim3(grid), dim3(NT), 0, s, B);
+  if (!PERSIST) grid = B.ntasks;
+  if (B.mixed) hipLaunchKernelGGL((k_rhs_batch<NN, EPB, float, float, PERSIST>), dim3(grid), dim3(NT), 0, s, B);
+  else hipLaunchKernelGGL((k_rhs_batch<NN, EPB, double, double, PERSIST>), dim3(grid), dim3(NT), 0, s, B);
   return hipGetLastError();
 }
+// variant bit 0: elements-per-workgroup shape (EPB0 / EPB1); bit 1: one task per workgroup
 template <int NN>
 static hipError_t launch_batch_n(const BatchLaunch& B, int variant, int grid, hipStream_t s) {
-  if (variant == 0) return launch_batch_shape<NN, Variants<NN>::EPB0>(B, grid, s);
-  return launch_batch_shape<NN, Variants<NN>::EPB1>(B, grid, s);
+  switch (variant) {
+    case 0: return launch_batch_shape<NN, Variants<NN>::EPB0, true>(B, grid, s);
+    case 1: return launch_batch_shape<NN, Variants<NN>::EPB1, true>(B, grid, s);
+    case 2: return launch_batch_shape<NN, Variants<NN>::EPB0, false>(B, grid, s);
+    default: return launch_batch_shape<NN, Variants<NN>::EPB1, false>(B, grid, s);
+  }
 }
 
 hipError_t launch_rhs_batch(const BatchLaunch& B, int n, int variant, int grid, hipStream_t s) {
@@ -1113,14 +1122,15 @@ static int resident_shape(int mixed) {
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   constexpr int NT = Geo<NN, EPB>::NT;
-  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, float, float>, NT, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, double, double>, NT, 0);
+  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, float, float, true>, NT, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, double, double, true>, NT, 0);
   if (e != hipSuccess || per < 1) per = 1;
   return cus * per;
 }
 template <int NN>
 static int resident_n(int variant, int mixed) {
-  return variant == 0 ? resident_shape<NN, Variants<NN>::EPB0>(mixed) : resident_shape<NN, Variants<NN>::EPB1>(mixed);
+  return (variant & 1) == 0 ? resident_shape<NN, Variants<NN>::EPB0>(mixed)
+                            : resident_shape<NN, Variants<NN>::EPB1>(mixed);
 }
 
 int batch_max_grid(int n, int variant, int mixed) {

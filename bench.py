@@ -40,7 +40,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--variant", type=int, default=-1,
                     help="pin the stage-kernel variant 0..3 (default: autotune), e.g. for profiled runs")
-    ap.add_argument("--xcd", type=int, default=1, help="XCD-contiguous numbering when --variant is pinned")
+    ap.add_argument("--xcd", type=int, default=1, help="workgroup order 0..3 when --variant is pinned")
+    ap.add_argument("--batch-steps", type=int, default=20, help="timed steps of the batched workload")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-solve workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -153,6 +155,52 @@ def compare_precisions(pk, p, steps, warmup):
     )
 
 
+def batch_run(pk, steps, warmup=1):
+    """The batched workload: 256 seeded unit-cube solves (4..16 elements per side, N 2..6), FP32
+    arithmetic / FP64 state, per-order persistent launches replayed from step graphs; accuracy of
+    every solve against the same batch in FP64 at the end."""
+    import torch
+    ps = pk.problem.config_c4(steps=steps)
+    descs = [pk.problem.batch_solve(p) for p in ps]
+    out = dict(workload="256 unit-cube solves, 4..16 elements per side, N 2..6 (mt19937_64 seed 0), "
+                        "Gaussian pulse, 2-element sponge, LSRK54 CFL 0.3, FP32 arithmetic / FP64 state")
+    runs = {}
+    for name, prec in (("mixed", pk.dg3d.MIXED), ("fp64", pk.dg3d.FP64)):
+        b = pk.dg3d.Batch(descs, precision=prec)
+        b.autotune()
+        b.step(warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.step(steps)
+        el = time.perf_counter() - t0
+        info = b.info()
+        stage_ms, _ = b.last_step_timing()
+        runs[name] = dict(b=b, elapsed=el, info=info, stage_ms=stage_ms)
+    nodes = runs["mixed"]["info"]["total_nodes"]
+    worst = 0.0
+    for i in range(len(ps)):
+        a = runs["mixed"]["b"].get_state(i).reshape(4, -1)
+        r = runs["fp64"]["b"].get_state(i).reshape(4, -1)
+        for f in range(4):
+            nr = np.linalg.norm(r[f])
+            if nr > 0:
+                worst = max(worst, float(np.linalg.norm(a[f] - r[f]) / nr))
+    for r in runs.values():
+        r["b"].close()
+    m, d = runs["mixed"], runs["fp64"]
+    out.update(
+        solves=len(ps), nodes=nodes, steps=steps,
+        dof_stage_per_s=nodes * 5 * steps / m["elapsed"], stage_ms=m["stage_ms"],
+        hbm_frac=nodes * BYTES_MIXED / (m["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        fp64_dof_stage_per_s=nodes * 5 * steps / d["elapsed"], fp64_stage_ms=d["stage_ms"],
+        fp64_hbm_frac=nodes * BYTES_FP64 / (d["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        mixed_rel_l2_max_vs_fp64=worst,
+        orders=m["info"]["orders"], variants=m["info"]["variants"], tasks=m["info"]["tasks"],
+        grids=m["info"]["grids"],
+    )
+    return out
+
+
 def cpu_baseline(p, steps):
     """The CPU oracle (per-line dense solves, OpenMP over elements) on `steps` LSRK54 steps of the
     same workload, on this host's cores."""
@@ -240,6 +288,8 @@ def main():
                                              args.steps, args.warmup)
         out["mixed_c3"] = compare_precisions(pk, pk.problem.config_c3(num_elements=args.elements, steps=args.steps),
                                              args.steps, args.warmup)
+    if not args.no_batch and not args.no_extra:
+        out["batch_c4"] = batch_run(pk, args.batch_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pk.problem.config_c2(num_elements=args.elements), args.cpu_steps)
     print(json.dumps(out))
