@@ -56,7 +56,7 @@ struct dg3d_batch {
   int64_t total_slots = 0;
   int cur = 0;                         // arena holding the state
   bool halos_pending = true;           // the next stage is the first of the run
-  dg3d::BatchSolve* d_solves = nullptr;
+  dg3d::RhsLaunch* d_records = nullptr;  // [solve][kBatchRecords]
   // per order present (descending n): task lists, grids and the tuned shape
   struct Group {
     int n = 0;
@@ -110,7 +110,7 @@ void free_batch(dg3d_batch* b) {
   hipFree(b->arena[0]);
   hipFree(b->arena[1]);
   hipFree(b->du);
-  hipFree(b->d_solves);
+  hipFree(b->d_records);
   for (auto& g : b->groups) {
     for (auto* t : g.d_tasks) hipFree(t);
     if (g.done) hipEventDestroy(g.done);
@@ -126,16 +126,11 @@ void free_batch(dg3d_batch* b) {
 dg3d::BatchLaunch stage_launch(const dg3d_batch* b, const dg3d_batch::Group& g, int variant, int stage, bool halo,
                                int from) {
   dg3d::BatchLaunch B{};
-  B.solves = b->d_solves;
+  B.records = b->d_records;
   B.tasks = g.d_tasks[variant];
   B.ntasks = g.ntasks[variant];
+  B.record = (halo && stage == 0) ? 10 + from : 2 * stage + from;
   B.mixed = (b->precision == DG3D_MIXED) ? 1 : 0;
-  B.use_halo = halo ? 1 : 0;
-  B.q_in = b->arena[from];
-  B.q_out = b->arena[1 - from];
-  B.du = b->du;
-  B.a = dg3d::kLsA[stage];
-  B.b = dg3d::kLsB[stage];
   return B;
 }
 
@@ -303,15 +298,14 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
       hipMemcpy(oo.opf[k], f.data(), sizeof(float) * cnt, hipMemcpyHostToDevice);
     }
   }
-  // per-solve device data and descriptors
-  std::vector<dg3d::BatchSolve> ds(count);
+  // per-solve device data and launch records
+  std::vector<dg3d::RhsLaunch> recs((size_t)count * dg3d::kBatchRecords);
   std::vector<double> stage;
   for (int i = 0; i < count; ++i) {
     const dg3d_solve_desc& d = descs[i];
     HostSolve& h = b->solves[i];
-    dg3d::BatchSolve& s = ds[i];
-    std::memset(&s, 0, sizeof(s));
-    dg3d::MeshArgs& m = s.mesh;
+    dg3d::RhsLaunch base{};
+    dg3d::MeshArgs& m = base.mesh;
     for (int a = 0; a < 3; ++a) {
       m.nE[a] = h.nE[a];
       m.periodic[a] = h.periodic[a];
@@ -351,8 +345,8 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
     b->allocs.push_back(mat);
     hipMemcpy(mat, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice);
     m.mat = mat;
+    const double* traces[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     for (int f = 0; f < 6; ++f) {
-      s.trace[f] = nullptr;
       if (!d.halo[f] || h.periodic[2 - f / 2]) continue;
       const int axis = 2 - f / 2;
       const int64_t fe = (int64_t)h.nE[axis == 0 ? 1 : 0] * h.nE[axis == 2 ? 1 : 2];
@@ -361,17 +355,32 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
       if (bmalloc(&p, cnt) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: halo allocation failed");
       b->allocs.push_back(p);
       hipMemcpy(p, d.halo[f], sizeof(double) * cnt, hipMemcpyHostToDevice);
-      s.trace[f] = p;
+      traces[f] = p;
       h.has_halo[f] = true;
     }
     const OrderOps& oo = b->ops[h.n];
+    base.n = h.n;
+    base.mixed = (precision == DG3D_MIXED) ? 1 : 0;
+    base.epilogue = dg3d::EPI_LSRK;
+    base.mask = dg3d::kTermAll;
+    base.ext.src_face = -1;
     for (int k = 0; k < 3; ++k) {
-      s.opd[k] = oo.opd[k];
-      s.opf[k] = oo.opf[k];
+      base.opd[k] = oo.opd[k];
+      base.opf[k] = oo.opf[k];
     }
-    s.slot0 = h.slot0;
-    s.dt = h.dt;
-    s.n = h.n;
+    base.dt = h.dt;
+    char* du0 = static_cast<char*>(b->du) + (precision == DG3D_MIXED ? 4 : 8) * (size_t)h.slot0;
+    base.du = du0;
+    for (int r = 0; r < dg3d::kBatchRecords; ++r) {
+      const int from = r & 1, st = r < 10 ? r / 2 : 0;
+      dg3d::RhsLaunch L = base;
+      L.q_in = b->arena[from] + h.slot0;
+      L.q_out = b->arena[1 - from] + h.slot0;
+      L.a = dg3d::kLsA[st];
+      L.b = dg3d::kLsB[st];
+      for (int f = 0; f < 6; ++f) L.ext.trace[f] = (r >= 10) ? traces[f] : nullptr;
+      recs[(size_t)i * dg3d::kBatchRecords + r] = L;
+    }
     // initial state: host [4][E][np] -> padded slots
     stage.assign((size_t)4 * h.fs, 0.0);
     for (int f = 0; f < 4; ++f)
@@ -381,8 +390,8 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
     e = hipMemcpy(b->arena[0] + h.slot0, stage.data(), sizeof(double) * stage.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(DG3D_E_HIP, std::string("batch: state upload: ") + hipGetErrorString(e));
   }
-  if (bmalloc(&b->d_solves, ds.size()) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: descriptor allocation failed");
-  hipMemcpy(b->d_solves, ds.data(), sizeof(dg3d::BatchSolve) * ds.size(), hipMemcpyHostToDevice);
+  if (bmalloc(&b->d_records, recs.size()) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: record allocation failed");
+  hipMemcpy(b->d_records, recs.data(), sizeof(dg3d::RhsLaunch) * recs.size(), hipMemcpyHostToDevice);
   // one group per order present, largest first; task lists for both shapes
   std::vector<int> orders;
   for (const HostSolve& h : b->solves)

@@ -110,33 +110,22 @@ hipError_t launch_rhs(const RhsLaunch& L, hipStream_t s);
 constexpr int kNumVariants = 4;
 constexpr int kNumOrders = 4;
 
-// Batched stages: many independent blocks (solves) of different sizes and orders advanced by one
-// persistent launch per LSRK54 stage. Each solve owns a region of 4 * mesh.fs slots in the
-// state arenas, starting at slot0; its element groups are tasks that the workgroups take in
-// turn.
-struct BatchSolve {
-  MeshArgs mesh;             // the solve's block (its own inv_w and material arrays)
-  int64_t slot0;             // first slot of the solve's region in the arenas
-  const double* trace[6];    // halo traces for the first stage of a run, or null
-  const double* opd[3];      // line operators of the solve's order
-  const float* opf[3];
-  double dt;
-  int n;
-};
+// Batched stages: many independent blocks (solves) of different sizes and orders. Each solve owns
+// a region of 4 * fs slots in the state arenas; its stage launches are described by launch records
+// (RhsLaunch with the solve's mesh, operators and field pointers), kBatchRecords per solve:
+// record 2 * stage + from for LSRK54 stage 0..4 reading arena `from`, and 10 + from for stage 0
+// with the first-stage halo traces. The solves' element groups are tasks that workgroups take.
+constexpr int kBatchRecords = 12;
 struct BatchTask {
   int solve;
   int e0;  // first element of the group
 };
 struct BatchLaunch {
-  const BatchSolve* solves;
-  const BatchTask* tasks;
+  const RhsLaunch* records;  // [solve][kBatchRecords]
+  const BatchTask* tasks;    // this order's groups
   int ntasks;
-  int mixed;         // 0: FP64 arithmetic and du; 1: FP32 arithmetic and du
-  int use_halo;      // 1 on the first stage of a run
-  const double* q_in;
-  double* q_out;
-  void* du;
-  double a, b;       // LSRK54 stage coefficients
+  int record;                // record index of this stage
+  int mixed;
 };
 // Per order, autotuned: bit 0 picks the elements-per-workgroup shape (the stage kernel's EPB0 or
 // EPB1), bit 1 one task per workgroup instead of the persistent loop.

@@ -682,64 +682,35 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
 }
 
 // ---- batched persistent stage kernels ---------------------------------------------------
-// One kernel per order: a persistent loop over the element groups (tasks) of every solve of
-// that order in the batch. Workgroup w takes tasks w, w + grid, ... (a static schedule: every
-// workgroup leaves the loop after the last task). The descriptors do not change during the
-// launch and are read through the constant address space (scalar loads the compiler may repeat
-// instead of holding them in registers).
-using ConstSolvePtr = const __attribute__((address_space(4))) BatchSolve*;
+// One kernel per order over the element groups (tasks) of every solve of that order in the batch.
+// Each solve's stage parameters are a launch record prepared on the host (one per stage and
+// arena parity), read through the constant address space. PERSIST: workgroup w takes tasks w,
+// w + grid, ... (a static schedule: every workgroup leaves the loop after the last task);
+// otherwise one task per workgroup.
+using ConstLaunchPtr = const __attribute__((address_space(4))) RhsLaunch*;
 using ConstTaskPtr = const __attribute__((address_space(4))) BatchTask*;
 
-__device__ __forceinline__ MeshArgs load_mesh(ConstSolvePtr sd) {
-  MeshArgs m;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    m.nE[a] = sd->mesh.nE[a];
-    m.periodic[a] = sd->mesh.periodic[a];
-    m.stride[a] = sd->mesh.stride[a];
-    m.inv_w[a] = sd->mesh.inv_w[a];
-  }
-  m.E = sd->mesh.E;
-  m.fs = sd->mesh.fs;
-  m.es = sd->mesh.es;
-  m.div12.m = sd->mesh.div12.m;
-  m.div12.s = sd->mesh.div12.s;
-  m.div2.m = sd->mesh.div2.m;
-  m.div2.s = sd->mesh.div2.s;
-  m.mat = sd->mesh.mat;
-  return m;
-}
-
-// PERSIST = false is the same kernel with one task per workgroup (grid = task count), for the
-// autotuner to weigh against the persistent form.
 template <int NN, int EPB, typename CT, typename DUT, bool PERSIST>
 __global__ void __launch_bounds__((Geo<NN, EPB>::NT)) k_rhs_batch(const BatchLaunch B) {
   __shared__ StageSmem<NN, EPB, CT, 0> sm;
   const ConstTaskPtr tasks = (ConstTaskPtr)B.tasks;
-  for (int t = blockIdx.x; t < B.ntasks; t += (PERSIST ? gridDim.x : B.ntasks)) {
-    if (PERSIST) __syncthreads();  // the previous task's phase 3 still reads LDS
-    const int solve = tasks[t].solve, e0 = tasks[t].e0;
-    const ConstSolvePtr sd = (ConstSolvePtr)(B.solves + solve);
-    RhsLaunch L{};
-    L.n = NN;
-    L.mixed = B.mixed;
-    L.epilogue = EPI_LSRK;
-    L.mask = kTermAll;
-    L.ext.src_face = -1;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      L.opd[i] = sd->opd[i];
-      L.opf[i] = sd->opf[i];
+  if constexpr (PERSIST) {
+    for (int t = blockIdx.x; t < B.ntasks; t += gridDim.x) {
+      __syncthreads();  // the previous task's phase 3 still reads LDS
+      // the lane index is re-read inside the loop: without it the compiler hoists the
+      // task-independent lane arithmetic out of the loop and holds it in registers across tasks
+      int tid;
+      asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"((int)threadIdx.x));
+      const int solve = tasks[t].solve, e0 = tasks[t].e0;
+      const RhsLaunch& L = *(const RhsLaunch*)((ConstLaunchPtr)(B.records + (int64_t)solve * kBatchRecords + B.record));
+      stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI_LSRK, DUT, 0>(L, L.mesh, L.ext.trace, e0, tid, sm);
     }
-    L.q_in = B.q_in + sd->slot0;
-    L.q_out = B.q_out + sd->slot0;
-    L.du = static_cast<DUT*>(B.du) + sd->slot0;
-    L.a = B.a;
-    L.b = B.b;
-    L.dt = sd->dt;
-    const MeshArgs m = load_mesh(sd);
-    const double* const* traces = B.use_halo ? (const double* const*)(B.solves + solve)->trace : nullptr;
-    stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI_LSRK, DUT, 0>(L, m, traces, e0, (int)threadIdx.x, sm);
+  } else {
+    const int t = blockIdx.x;
+    if (t >= B.ntasks) return;
+    const int solve = tasks[t].solve, e0 = tasks[t].e0;
+    const RhsLaunch& L = *(const RhsLaunch*)((ConstLaunchPtr)(B.records + (int64_t)solve * kBatchRecords + B.record));
+    stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI_LSRK, DUT, 0>(L, L.mesh, L.ext.trace, e0, (int)threadIdx.x, sm);
   }
 }
 
