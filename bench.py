@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--xcd", type=int, default=1, help="workgroup order 0..3 when --variant is pinned")
     ap.add_argument("--batch-steps", type=int, default=20, help="timed steps of the batched workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-solve workload")
+    ap.add_argument("--c5-steps", type=int, default=2000, help="steps of the combined 48^3 N=5 workload (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -109,14 +110,15 @@ def second_harmonic(trace, dt, f0):
     return float(np.abs(np.sum(trace * np.exp(-2j * np.pi * 2 * f0 * t))) * 2.0 / max(len(trace), 1))
 
 
-def compare_precisions(pk, p, steps, warmup):
-    """FP64 and mixed runs of the same problem: speed and accuracy of mixed at t_end."""
+def compare_precisions(pk, p, steps, warmup, mixed_contraction=0, matrix_core_runs=True):
+    """FP64 and mixed runs of the same problem: speed and accuracy of mixed at t_end (the mixed
+    run's derivative contraction: 0 VALU, 1 MFMA FP32, 2 MFMA split-BF16)."""
     pr = probe_node(p)
     ctx64, r64 = timed_run(pk, p, pk.dg3d.FP64, steps, warmup, probe=pr)
     s64 = ctx64.get_state(0)
     tr64 = ctx64.get_probes()[:, 0]
     ctx64.close()
-    ctx32, r32 = timed_run(pk, p, pk.dg3d.MIXED, steps, warmup, probe=pr)
+    ctx32, r32 = timed_run(pk, p, pk.dg3d.MIXED, steps, warmup, probe=pr, contraction=mixed_contraction)
     s32 = ctx32.get_state(0)
     tr32 = ctx32.get_probes()[:, 0]
     ctx32.close()
@@ -135,6 +137,8 @@ def compare_precisions(pk, p, steps, warmup):
     rate32 = p.dof * 5 * steps / r32["elapsed"]
     matrix_core = {}
     for name, con in (("mfma_f32", pk.dg3d.CONTRACT_MFMA_F32), ("mfma_bf16x3", pk.dg3d.CONTRACT_MFMA_BF16)):
+        if not matrix_core_runs:
+            break
         ctxm, rm = timed_run(pk, p, pk.dg3d.MIXED, steps, warmup, contraction=con)
         sm = ctxm.get_state(0).reshape(4, -1)
         ctxm.close()
@@ -143,6 +147,7 @@ def compare_precisions(pk, p, steps, warmup):
                                  speedup_vs_fp64=r64["stage_ms"] / rm["stage_ms"], rel_l2_max=max(relm))
     return dict(
         workload=p.name, elements=list(p.num_elements), order=p.order, dof=p.dof, steps=steps,
+        mixed_contraction=["valu", "mfma_f32", "mfma_bf16x3"][mixed_contraction],
         fp64_dof_stage_per_s=rate64, mixed_dof_stage_per_s=rate32, mixed_speedup=rate32 / rate64,
         fp64_stage_ms=r64["stage_ms"], mixed_stage_ms=r32["stage_ms"],
         fp64_variant=[r64["variant"], r64["xcd"]], mixed_variant=[r32["variant"], r32["xcd"]],
@@ -290,6 +295,12 @@ def main():
                                              args.steps, args.warmup)
     if not args.no_batch and not args.no_extra:
         out["batch_c4"] = batch_run(pk, args.batch_steps)
+    if not args.no_extra and args.c5_steps > 0:
+        # combined workload: stretched two-layer 48^3 N=5 with a focused burst; mixed with the
+        # split-BF16 matrix-core contraction against FP64
+        out["combined_c5"] = compare_precisions(pk, pk.problem.config_c5(steps=args.c5_steps), args.c5_steps,
+                                                args.warmup, mixed_contraction=pk.dg3d.CONTRACT_MFMA_BF16,
+                                                matrix_core_runs=False)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pk.problem.config_c2(num_elements=args.elements), args.cpu_steps)
     print(json.dumps(out))
