@@ -27,20 +27,45 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build_library(verbose_resources=False):
+def _run_parallel(cmds, jobs):
+    """Run the commands, at most `jobs` at a time; raise if any fails (after all have ended)."""
+    pending = list(cmds)
+    running = []
+    failed = []
+    while pending or running:
+        while pending and len(running) < jobs:
+            cmd = pending.pop(0)
+            print("+", " ".join(cmd), flush=True)
+            running.append((cmd, subprocess.Popen(cmd)))
+        cmd, proc = running.pop(0)
+        if proc.wait() != 0:
+            failed.append(cmd)
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
+
+
+# The kernel source is compiled once per order (the stage kernels of order n-1 with
+# -DDG3D_TU_ORDER=n) and once for the order dispatch and the order-independent kernels.
+KERNEL_UNITS = [("common", [])] + [(f"n{n}", [f"-DDG3D_TU_ORDER={n}"]) for n in range(2, 9)]
+
+
+def build_library(verbose_resources=False, jobs=None):
     objdir = os.path.join(PKG, "_obj")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
-    for src in ["dg3d_kernels.hip"]:
-        obj = os.path.join(objdir, src + ".o")
-        extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
-        _run([HIPCC, "-c", f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, *extra,
-              os.path.join(CSRC, src), "-o", obj])
+    jobs = jobs or max(1, min(8, os.cpu_count() or 1))
+    extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
+    cmds, objs = [], []
+    for tag, defs in KERNEL_UNITS:
+        obj = os.path.join(objdir, f"dg3d_kernels_{tag}.o")
+        cmds.append([HIPCC, "-c", f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, *defs, *extra,
+                     os.path.join(CSRC, "dg3d_kernels.hip"), "-o", obj])
         objs.append(obj)
     for src in ["dg3d_capi.cpp", "dg3d_batch.cpp", "dg3d_setup.cpp", "basis.cpp"]:
         obj = os.path.join(objdir, src + ".o")
-        _run([HIPCC, "-x", "c++", "-c", *COMMON, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC, "-x", "c++", "-c", *COMMON, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                     os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    _run_parallel(cmds, jobs)
     out = os.path.join(PKG, "libdg3d.so")
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", *objs, "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)

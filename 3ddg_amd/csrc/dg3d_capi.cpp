@@ -267,60 +267,70 @@ int reduce_partials(dg3d_ctx* c, int per_block, int blocks) {
   return DG3D_OK;
 }
 
-// Times every (kernel variant, XCD remap) pair on launches of a stage that reads and writes the
-// full register set (LSRK54 stage 2; RK33 stage 2 with register 0 as both inputs). They write
-// only the work registers, which the next step's first stage overwrites without reading, so
-// tuning leaves the solution unchanged. The fastest pair is kept.
+// Times every (kernel variant, workgroup order) pair on launches of a stage that reads and writes
+// the full register set (LSRK54 stage 2; RK33 stage 2 with register 0 as both inputs), after a
+// warm-up, in two sweeps of which each pair keeps the faster. The launches write only the work
+// registers, which the next step's first stage overwrites without reading, so tuning leaves the
+// solution unchanged. The fastest pair is kept.
 int autotune(dg3d_ctx* c, int scheme) {
   int rc = ensure_work(c, scheme);
   if (rc) return rc;
-  double best = 1e300;
-  int bv = 0, bx = c->xcd;
-  const int reps = 4;
   if (scheme == DG3D_LSRK54) {  // give the du register defined contents before it is read
     hipError_t e = hipMemsetAsync(c->work_du, 0, (c->precision == DG3D_MIXED ? 4 : 8) * 4 * (size_t)c->fs, c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "autotune");
   }
-  for (int v = 0; v < dg3d::kNumVariants; ++v) {
-    for (int x = 0; x < dg3d::kNumOrders; ++x) {
-      dg3d::RhsLaunch L = base_launch(c);
-      L.variant = v;
-      L.xcd_remap = x;
-      L.mixed = (c->precision == DG3D_MIXED) ? 1 : 0;
-      L.mask = dg3d::kTermAll;
-      L.dt = 1e-30;
-      L.q_in = c->regs[0].d;
-      if (scheme == DG3D_LSRK54) {
-        L.epilogue = dg3d::EPI_LSRK;
-        L.q_out = c->work_q;
-        L.du = c->work_du;
-        L.a = kLsA[1];
-        L.b = kLsB[1];
-      } else {
-        L.epilogue = dg3d::EPI_RK33;
-        L.old = c->regs[0].d;
-        L.out = c->work_k2;
-        L.a = 0.75;
-        L.b = 0.25;
-      }
-      L.ext = c->ext_for(-1, 0.0);
-      hipError_t e = dg3d::launch_rhs(L, c->stream);  // warm-up (and first-use code load)
-      if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
-      for (int r = 0; r < reps && e == hipSuccess; ++r) e = dg3d::launch_rhs(L, c->stream);
-      if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
-      if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
-      if (e != hipSuccess) return c->hip_fail(e, "autotune");
-      float ms = 0.f;
-      hipEventElapsedTime(&ms, c->ev0, c->ev1);
-      if (ms < best) {
-        best = ms;
-        bv = v;
-        bx = x;
+  auto tuning_launch = [&](int v, int x) {
+    dg3d::RhsLaunch L = base_launch(c);
+    L.variant = v;
+    L.xcd_remap = x;
+    L.mixed = (c->precision == DG3D_MIXED) ? 1 : 0;
+    L.mask = dg3d::kTermAll;
+    L.dt = 1e-30;
+    L.q_in = c->regs[0].d;
+    if (scheme == DG3D_LSRK54) {
+      L.epilogue = dg3d::EPI_LSRK;
+      L.q_out = c->work_q;
+      L.du = c->work_du;
+      L.a = kLsA[1];
+      L.b = kLsB[1];
+    } else {
+      L.epilogue = dg3d::EPI_RK33;
+      L.old = c->regs[0].d;
+      L.out = c->work_k2;
+      L.a = 0.75;
+      L.b = 0.25;
+    }
+    L.ext = c->ext_for(-1, 0.0);
+    return L;
+  };
+  // bring the clocks up before anything is timed
+  hipError_t e = hipSuccess;
+  for (int r = 0; r < 8 && e == hipSuccess; ++r) e = dg3d::launch_rhs(tuning_launch(0, c->xcd), c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "autotune");
+  // two sweeps over every (variant, order) pair; each pair keeps its faster sweep
+  constexpr int kPairs = dg3d::kNumVariants * dg3d::kNumOrders;
+  std::vector<float> best(kPairs, 1e30f);
+  const int reps = 4;
+  for (int sweep = 0; sweep < 2; ++sweep) {
+    for (int v = 0; v < dg3d::kNumVariants; ++v) {
+      for (int x = 0; x < dg3d::kNumOrders; ++x) {
+        const dg3d::RhsLaunch L = tuning_launch(v, x);
+        e = dg3d::launch_rhs(L, c->stream);  // first use loads the code object
+        if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = dg3d::launch_rhs(L, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+        if (e != hipSuccess) return c->hip_fail(e, "autotune");
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        float& b = best[v * dg3d::kNumOrders + x];
+        b = std::min(b, ms);
       }
     }
   }
-  c->tune_variant[scheme] = bv;
-  c->tune_xcd[scheme] = bx;
+  const int k = (int)(std::min_element(best.begin(), best.end()) - best.begin());
+  c->tune_variant[scheme] = k / dg3d::kNumOrders;
+  c->tune_xcd[scheme] = k % dg3d::kNumOrders;
   return DG3D_OK;
 }
 

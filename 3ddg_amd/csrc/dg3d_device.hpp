@@ -92,7 +92,7 @@ struct RhsLaunch {
   int epilogue;     // Epilogue
   unsigned mask;    // term mask
   int xcd_remap;    // workgroup order 0..kNumOrders-1 (autotuned)
-  int variant;      // kernel shape variant 0..kNumVariants-1 (autotuned)
+  int variant;      // kernel variant 0..kNumVariants-1 (autotuned)
   int contraction;  // mixed path: 0 VALU, 1 MFMA FP32, 2 MFMA split-BF16
   MeshArgs mesh;
   ExtArgs ext;
@@ -107,7 +107,7 @@ struct RhsLaunch {
 };
 
 hipError_t launch_rhs(const RhsLaunch& L, hipStream_t s);
-constexpr int kNumVariants = 4;
+constexpr int kNumVariants = 8;
 constexpr int kNumOrders = 4;
 
 // Batched stages: many independent blocks (solves) of different sizes and orders. Each solve owns

@@ -35,7 +35,7 @@ struct Shape : Geo<NN, (NN == 2) ? 32 : (NN == 3) ? 8 : (NN == 4) ? 4 : (NN == 5
 
 // Tunable variants of the stage kernel (chosen at run time by the autotuner of the C ABI):
 // bit 0 selects the alternative elements-per-workgroup, bit 1 the occupancy target (minimum
-// waves per SIMD the register allocation must allow).
+// waves per SIMD the register allocation must allow), bit 2 late RK-register loads.
 template <int NN>
 struct Variants {
   static constexpr int EPB0 = Shape<NN>::EPB;
@@ -391,8 +391,10 @@ struct StageSmem {
 
 // One workgroup's share of a stage: elements e0 .. e0+EPB-1 of the block described by m (field
 // arrays in L; traces: the 6 exterior face traces in memory, or null for none). NT is the
-// workgroup size (>= EPB * ES); every lane must call it, since it holds barriers.
-template <int NN, int EPB, int NT, typename CT, int EPI, typename DUT, int CONTRACT>
+// workgroup size (>= EPB * ES); every lane must call it, since it holds barriers. LATE issues the
+// RK-register loads (du / old, needed only by the epilogue) after the first barrier instead of
+// with the rest: fewer registers live at the peak, hence more waves, against later loads.
+template <int NN, int EPB, int NT, typename CT, int EPI, typename DUT, int CONTRACT, bool LATE = false>
 __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m, const double* const* traces,
                                            const int e0, const int tid, StageSmem<NN, EPB, CT, CONTRACT>& sm) {
   using S = Geo<NN, EPB>;
@@ -435,17 +437,22 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   if constexpr (EPI == EPI_LSRK) {
 #pragma unroll
     for (int f = 0; f < 4; ++f) rp[f] = make_rsrc(static_cast<const DUT*>(L.du) + f * fs, fbdu);
-    if (L.a != 0.0) {
-#pragma unroll
-      for (int f = 0; f < 4; ++f) pre[f] = (double)bload<DUT, kPolicyNT>(rp[f], gi);
-    }
   }
-  if constexpr (EPI == EPI_RK33) {
-    if (L.a != 0.0) {
+  auto load_pre = [&]() {
+    if constexpr (EPI == EPI_LSRK) {
+      if (L.a != 0.0) {
 #pragma unroll
-      for (int f = 0; f < 4; ++f) pre[f] = bload<double, kPolicyNT>(make_rsrc(L.old + f * fs, fb64), gi);
+        for (int f = 0; f < 4; ++f) pre[f] = (double)bload<DUT, kPolicyNT>(rp[f], gi);
+      }
     }
-  }
+    if constexpr (EPI == EPI_RK33) {
+      if (L.a != 0.0) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) pre[f] = bload<double, kPolicyNT>(make_rsrc(L.old + f * fs, fb64), gi);
+      }
+    }
+  };
+  if constexpr (!LATE) load_pre();
   const Rsrc rm = make_rsrc(m.mat, (uint32_t)(m.E * kMatStride * 8));
   const uint32_t mi = (uint32_t)(valid ? e : 0) * kMatStride;
   const CT c = (CT)bload<double>(rm, mi + MAT_C), r0 = (CT)bload<double>(rm, mi + MAT_RHO0);
@@ -558,6 +565,8 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   }
   __syncthreads();
 
+  if constexpr (LATE) load_pre();
+
   // ---- phase 2: upwind flux pair of every face node -------------------------------------
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -668,7 +677,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   }
 }
 
-template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT, int CONTRACT = 0>
+template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT, int CONTRACT = 0, bool LATE = false>
 __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunch L) {
   __shared__ StageSmem<NN, EPB, CT, CONTRACT> sm;
   const MeshArgs& m = L.mesh;
@@ -677,8 +686,8 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
     grp = tiled_group(remap_block(blockIdx.x, gridDim.x, 1, 1), m.nE[0], m.nE[1], m.nE[2], EPB);
   else
     grp = remap_block(blockIdx.x, gridDim.x, L.xcd_remap, order_run(m.nE[2], EPB));
-  stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI, DUT, CONTRACT>(L, m, L.ext.trace, (int)(grp * EPB),
-                                                                (int)threadIdx.x, sm);
+  stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI, DUT, CONTRACT, LATE>(L, m, L.ext.trace, (int)(grp * EPB),
+                                                                      (int)threadIdx.x, sm);
 }
 
 // ---- batched persistent stage kernels ---------------------------------------------------
@@ -714,6 +723,7 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT)) k_rhs_batch(const BatchLau
   }
 }
 
+#ifndef DG3D_TU_ORDER
 // Exterior trace of the source face at one stage time: the face's halo trace (or zero) plus the
 // boundary source, p = s(t + lead), rho' = s/c^2, u_axis = +-s/(rho0 c) entering the block.
 __global__ void k_source_trace(ExtArgs x, MeshArgs m, int n, const double* halo, double* out) {
@@ -740,6 +750,7 @@ __global__ void k_source_trace(ExtArgs x, MeshArgs m, int n, const double* halo,
   out[i * 3 + 1] = h1 + ((face & 1) ? -1.0 : 1.0) * sv / (rd * cd);
   out[i * 3 + 2] = h2;
 }
+#endif  // DG3D_TU_ORDER
 
 // LDG operators for one axis (Element.cpp:623-758), FP64, one lane per node.
 template <int NN>
@@ -811,6 +822,7 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
   }
 }
 
+#ifndef DG3D_TU_ORDER
 __global__ void k_axpby(double* out, double a, const double* in, double b, int64_t count) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -921,8 +933,9 @@ __global__ void k_probe(const double* reg, MeshArgs m, int n, const int64_t* nod
   const double r = reg[g];
   out[i] = c * c * (r + ba * 0.5 / r0 * r * r);
 }
+#endif  // DG3D_TU_ORDER
 
-template <int NN, int EPB, int MINW>
+template <int NN, int EPB, int MINW, bool LATE>
 hipError_t launch_rhs_v(const RhsLaunch& L, hipStream_t s) {
   using S = Geo<NN, EPB>;
   const int64_t blocks = (L.mesh.E + EPB - 1) / EPB;
@@ -932,29 +945,35 @@ hipError_t launch_rhs_v(const RhsLaunch& L, hipStream_t s) {
     if (L.mixed) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_ACCUM, double>), grid, block, 0, s, L);
   } else if (L.epilogue == EPI_LSRK) {
-    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_LSRK, double>), grid, block, 0, s, L);
+    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_LSRK, double, 0, LATE>), grid, block, 0, s, L);
     else if (L.contraction == 1) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float, 1>), grid, block, 0, s, L);
     else if (L.contraction == 2) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float, 2>), grid, block, 0, s, L);
-    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float>), grid, block, 0, s, L);
+    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float, 0, LATE>), grid, block, 0, s, L);
   } else {
-    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_RK33, double>), grid, block, 0, s, L);
+    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_RK33, double, 0, LATE>), grid, block, 0, s, L);
     else if (L.contraction == 1) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double, 1>), grid, block, 0, s, L);
     else if (L.contraction == 2) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double, 2>), grid, block, 0, s, L);
-    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double>), grid, block, 0, s, L);
+    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double, 0, LATE>), grid, block, 0, s, L);
   }
   return hipGetLastError();
 }
 
+// Variant bits: 1 alternative elements per workgroup, 2 occupancy target, 4 late RK-register
+// loads (VALU contraction only; the matrix-core contractions ignore the bit).
 template <int NN>
 hipError_t launch_rhs_n(const RhsLaunch& L, hipStream_t s) {
   using V = Variants<NN>;
   // the accumulate epilogue (Block::Operate*) only uses the default shape
   const int v = (L.epilogue == EPI_ACCUM) ? 0 : L.variant;
   switch (v) {
-    case 0: return launch_rhs_v<NN, V::EPB0, 1>(L, s);
-    case 1: return launch_rhs_v<NN, V::EPB1, 1>(L, s);
-    case 2: return launch_rhs_v<NN, V::EPB0, V::MINW1>(L, s);
-    case 3: return launch_rhs_v<NN, V::EPB1, V::MINW1>(L, s);
+    case 0: return launch_rhs_v<NN, V::EPB0, 1, false>(L, s);
+    case 1: return launch_rhs_v<NN, V::EPB1, 1, false>(L, s);
+    case 2: return launch_rhs_v<NN, V::EPB0, V::MINW1, false>(L, s);
+    case 3: return launch_rhs_v<NN, V::EPB1, V::MINW1, false>(L, s);
+    case 4: return launch_rhs_v<NN, V::EPB0, 1, true>(L, s);
+    case 5: return launch_rhs_v<NN, V::EPB1, 1, true>(L, s);
+    case 6: return launch_rhs_v<NN, V::EPB0, V::MINW1, true>(L, s);
+    case 7: return launch_rhs_v<NN, V::EPB1, V::MINW1, true>(L, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -968,31 +987,142 @@ hipError_t launch_ldg_n(const LdgLaunch& L, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Per-order launchers. The stage kernels of one order are instantiated in a translation unit of
+// their own (this file built with DG3D_TU_ORDER = 2..8), so the orders compile in parallel; the
+// file built without it holds the dispatch on the order and the order-independent kernels.
+template <int NN, int EPB, bool PERSIST>
+hipError_t launch_batch_shape(const BatchLaunch& B, int grid, hipStream_t s) {
+  constexpr int NT = Geo<NN, EPB>::NT;
+  if (!PERSIST) grid = B.ntasks;
+  if (B.mixed) hipLaunchKernelGGL((k_rhs_batch<NN, EPB, float, float, PERSIST>), dim3(grid), dim3(NT), 0, s, B);
+  else hipLaunchKernelGGL((k_rhs_batch<NN, EPB, double, double, PERSIST>), dim3(grid), dim3(NT), 0, s, B);
+  return hipGetLastError();
+}
+// variant bit 0: elements-per-workgroup shape (EPB0 / EPB1); bit 1: one task per workgroup
+template <int NN>
+hipError_t launch_batch_n(const BatchLaunch& B, int variant, int grid, hipStream_t s) {
+  switch (variant) {
+    case 0: return launch_batch_shape<NN, Variants<NN>::EPB0, true>(B, grid, s);
+    case 1: return launch_batch_shape<NN, Variants<NN>::EPB1, true>(B, grid, s);
+    case 2: return launch_batch_shape<NN, Variants<NN>::EPB0, false>(B, grid, s);
+    default: return launch_batch_shape<NN, Variants<NN>::EPB1, false>(B, grid, s);
+  }
+}
+
+template <int NN, int EPB>
+int resident_shape(int mixed) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  constexpr int NT = Geo<NN, EPB>::NT;
+  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, float, float, true>, NT, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, double, double, true>, NT, 0);
+  if (e != hipSuccess || per < 1) per = 1;
+  return cus * per;
+}
+template <int NN>
+int resident_n(int variant, int mixed) {
+  return (variant & 1) == 0 ? resident_shape<NN, Variants<NN>::EPB0>(mixed)
+                            : resident_shape<NN, Variants<NN>::EPB1>(mixed);
+}
+
 }  // namespace
+
+#define DG3D_ORDER_API(n)                                                                     \
+  hipError_t launch_rhs_o##n(const RhsLaunch& L, hipStream_t s);                             \
+  hipError_t launch_ldg_o##n(const LdgLaunch& L, hipStream_t s);                             \
+  hipError_t launch_batch_o##n(const BatchLaunch& B, int variant, int grid, hipStream_t s);  \
+  int resident_o##n(int variant, int mixed);
+DG3D_ORDER_API(2)
+DG3D_ORDER_API(3)
+DG3D_ORDER_API(4)
+DG3D_ORDER_API(5)
+DG3D_ORDER_API(6)
+DG3D_ORDER_API(7)
+DG3D_ORDER_API(8)
+#undef DG3D_ORDER_API
+
+#ifdef DG3D_TU_ORDER
+#define DG3D_CAT2(a, b) a##b
+#define DG3D_CAT(a, b) DG3D_CAT2(a, b)
+hipError_t DG3D_CAT(launch_rhs_o, DG3D_TU_ORDER)(const RhsLaunch& L, hipStream_t s) {
+  return launch_rhs_n<DG3D_TU_ORDER>(L, s);
+}
+hipError_t DG3D_CAT(launch_ldg_o, DG3D_TU_ORDER)(const LdgLaunch& L, hipStream_t s) {
+  return launch_ldg_n<DG3D_TU_ORDER>(L, s);
+}
+hipError_t DG3D_CAT(launch_batch_o, DG3D_TU_ORDER)(const BatchLaunch& B, int variant, int grid, hipStream_t s) {
+  return launch_batch_n<DG3D_TU_ORDER>(B, variant, grid, s);
+}
+int DG3D_CAT(resident_o, DG3D_TU_ORDER)(int variant, int mixed) { return resident_n<DG3D_TU_ORDER>(variant, mixed); }
+#else
 
 hipError_t launch_rhs(const RhsLaunch& L, hipStream_t s) {
   switch (L.n) {
-    case 2: return launch_rhs_n<2>(L, s);
-    case 3: return launch_rhs_n<3>(L, s);
-    case 4: return launch_rhs_n<4>(L, s);
-    case 5: return launch_rhs_n<5>(L, s);
-    case 6: return launch_rhs_n<6>(L, s);
-    case 7: return launch_rhs_n<7>(L, s);
-    case 8: return launch_rhs_n<8>(L, s);
+    case 2: return launch_rhs_o2(L, s);
+    case 3: return launch_rhs_o3(L, s);
+    case 4: return launch_rhs_o4(L, s);
+    case 5: return launch_rhs_o5(L, s);
+    case 6: return launch_rhs_o6(L, s);
+    case 7: return launch_rhs_o7(L, s);
+    case 8: return launch_rhs_o8(L, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 hipError_t launch_ldg(const LdgLaunch& L, hipStream_t s) {
   switch (L.n) {
-    case 2: return launch_ldg_n<2>(L, s);
-    case 3: return launch_ldg_n<3>(L, s);
-    case 4: return launch_ldg_n<4>(L, s);
-    case 5: return launch_ldg_n<5>(L, s);
-    case 6: return launch_ldg_n<6>(L, s);
-    case 7: return launch_ldg_n<7>(L, s);
-    case 8: return launch_ldg_n<8>(L, s);
+    case 2: return launch_ldg_o2(L, s);
+    case 3: return launch_ldg_o3(L, s);
+    case 4: return launch_ldg_o4(L, s);
+    case 5: return launch_ldg_o5(L, s);
+    case 6: return launch_ldg_o6(L, s);
+    case 7: return launch_ldg_o7(L, s);
+    case 8: return launch_ldg_o8(L, s);
     default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_rhs_batch(const BatchLaunch& B, int n, int variant, int grid, hipStream_t s) {
+  if (B.ntasks <= 0) return hipSuccess;
+  if (grid < 1) grid = 1;
+  if (variant < 0 || variant >= kNumBatchVariants) return hipErrorInvalidValue;
+  switch (n) {
+    case 2: return launch_batch_o2(B, variant, grid, s);
+    case 3: return launch_batch_o3(B, variant, grid, s);
+    case 4: return launch_batch_o4(B, variant, grid, s);
+    case 5: return launch_batch_o5(B, variant, grid, s);
+    case 6: return launch_batch_o6(B, variant, grid, s);
+    case 7: return launch_batch_o7(B, variant, grid, s);
+    case 8: return launch_batch_o8(B, variant, grid, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int batch_max_grid(int n, int variant, int mixed) {
+  switch (n) {
+    case 2: return resident_o2(variant, mixed);
+    case 3: return resident_o3(variant, mixed);
+    case 4: return resident_o4(variant, mixed);
+    case 5: return resident_o5(variant, mixed);
+    case 6: return resident_o6(variant, mixed);
+    case 7: return resident_o7(variant, mixed);
+    case 8: return resident_o8(variant, mixed);
+    default: return 0;
+  }
+}
+
+int batch_epb(int variant, int n) {
+  const bool alt = (variant & 1) != 0;
+  switch (n) {
+    case 2: return alt ? Variants<2>::EPB1 : Variants<2>::EPB0;
+    case 3: return alt ? Variants<3>::EPB1 : Variants<3>::EPB0;
+    case 4: return alt ? Variants<4>::EPB1 : Variants<4>::EPB0;
+    case 5: return alt ? Variants<5>::EPB1 : Variants<5>::EPB0;
+    case 6: return alt ? Variants<6>::EPB1 : Variants<6>::EPB0;
+    case 7: return alt ? Variants<7>::EPB1 : Variants<7>::EPB0;
+    case 8: return alt ? Variants<8>::EPB1 : Variants<8>::EPB0;
+    default: return 0;
   }
 }
 
@@ -1038,90 +1168,13 @@ hipError_t launch_cfl(const MeshArgs& m, double dxi_half, double* part, hipStrea
   hipLaunchKernelGGL(k_cfl, dim3(reduce_blocks(m.E)), dim3(kRedThreads), 0, s, m, dxi_half, part);
   return hipGetLastError();
 }
-int batch_epb(int variant, int n) {
-  const bool alt = (variant & 1) != 0;
-  switch (n) {
-    case 2: return alt ? Variants<2>::EPB1 : Variants<2>::EPB0;
-    case 3: return alt ? Variants<3>::EPB1 : Variants<3>::EPB0;
-    case 4: return alt ? Variants<4>::EPB1 : Variants<4>::EPB0;
-    case 5: return alt ? Variants<5>::EPB1 : Variants<5>::EPB0;
-    case 6: return alt ? Variants<6>::EPB1 : Variants<6>::EPB0;
-    case 7: return alt ? Variants<7>::EPB1 : Variants<7>::EPB0;
-    case 8: return alt ? Variants<8>::EPB1 : Variants<8>::EPB0;
-    default: return 0;
-  }
-}
-
-template <int NN, int EPB, bool PERSIST>
-static hipError_t launch_batch_shape(const BatchLaunch& B, int grid, hipStream_t s) {
-  constexpr int NT = Geo<NN, EPB>::NT;
-  if (!PERSIST) grid = B.ntasks;
-  if (B.mixed) hipLaunchKernelGGL((k_rhs_batch<NN, EPB, float, float, PERSIST>), dim3(grid), dim3(NT), 0, s, B);
-  else hipLaunchKernelGGL((k_rhs_batch<NN, EPB, double, double, PERSIST>), dim3(grid), dim3(NT), 0, s, B);
-  return hipGetLastError();
-}
-// variant bit 0: elements-per-workgroup shape (EPB0 / EPB1); bit 1: one task per workgroup
-template <int NN>
-static hipError_t launch_batch_n(const BatchLaunch& B, int variant, int grid, hipStream_t s) {
-  switch (variant) {
-    case 0: return launch_batch_shape<NN, Variants<NN>::EPB0, true>(B, grid, s);
-    case 1: return launch_batch_shape<NN, Variants<NN>::EPB1, true>(B, grid, s);
-    case 2: return launch_batch_shape<NN, Variants<NN>::EPB0, false>(B, grid, s);
-    default: return launch_batch_shape<NN, Variants<NN>::EPB1, false>(B, grid, s);
-  }
-}
-
-hipError_t launch_rhs_batch(const BatchLaunch& B, int n, int variant, int grid, hipStream_t s) {
-  if (B.ntasks <= 0) return hipSuccess;
-  if (grid < 1) grid = 1;
-  if (variant < 0 || variant >= kNumBatchVariants) return hipErrorInvalidValue;
-  switch (n) {
-    case 2: return launch_batch_n<2>(B, variant, grid, s);
-    case 3: return launch_batch_n<3>(B, variant, grid, s);
-    case 4: return launch_batch_n<4>(B, variant, grid, s);
-    case 5: return launch_batch_n<5>(B, variant, grid, s);
-    case 6: return launch_batch_n<6>(B, variant, grid, s);
-    case 7: return launch_batch_n<7>(B, variant, grid, s);
-    case 8: return launch_batch_n<8>(B, variant, grid, s);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-template <int NN, int EPB>
-static int resident_shape(int mixed) {
-  int dev = 0, cus = 0, per = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  constexpr int NT = Geo<NN, EPB>::NT;
-  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, float, float, true>, NT, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, double, double, true>, NT, 0);
-  if (e != hipSuccess || per < 1) per = 1;
-  return cus * per;
-}
-template <int NN>
-static int resident_n(int variant, int mixed) {
-  return (variant & 1) == 0 ? resident_shape<NN, Variants<NN>::EPB0>(mixed)
-                            : resident_shape<NN, Variants<NN>::EPB1>(mixed);
-}
-
-int batch_max_grid(int n, int variant, int mixed) {
-  switch (n) {
-    case 2: return resident_n<2>(variant, mixed);
-    case 3: return resident_n<3>(variant, mixed);
-    case 4: return resident_n<4>(variant, mixed);
-    case 5: return resident_n<5>(variant, mixed);
-    case 6: return resident_n<6>(variant, mixed);
-    case 7: return resident_n<7>(variant, mixed);
-    case 8: return resident_n<8>(variant, mixed);
-    default: return 0;
-  }
-}
-
 hipError_t launch_probe(const double* reg, const MeshArgs& m, int n, const int64_t* nodes, int count, double* out,
                         hipStream_t s) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_probe, dim3((count + 63) / 64), dim3(64), 0, s, reg, m, n, nodes, count, out);
   return hipGetLastError();
 }
+
+#endif  // DG3D_TU_ORDER
 
 }  // namespace dg3d

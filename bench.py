@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--variant", type=int, default=-1,
-                    help="pin the stage-kernel variant 0..3 (default: autotune), e.g. for profiled runs")
+                    help="pin the stage-kernel variant 0..7 (default: autotune), e.g. for profiled runs")
     ap.add_argument("--xcd", type=int, default=1, help="workgroup order 0..3 when --variant is pinned")
     ap.add_argument("--batch-steps", type=int, default=20, help="timed steps of the batched workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-solve workload")

@@ -166,7 +166,8 @@ int dg3d_step(dg3d_ctx* ctx, int scheme, double dt, int nsteps, double* t);
 int dg3d_stage(dg3d_ctx* ctx, int scheme, int stage, double dt, double t);
 
 /* Autotuner of the fused stage kernel (no counterpart in the reference): times every kernel
- * shape variant (elements per workgroup x occupancy target, 0..3) under every workgroup order
+ * variant (0..7: elements per workgroup, occupancy target, RK-register loads issued with the
+ * node's own loads or after the first barrier) under every workgroup order
  * (0..3, as xcd_remap above) on launches with a later stage's coefficients that write only the
  * work registers, and keeps the fastest pair for `scheme`. dg3d_step runs it on its first call
  * unless a pair was pinned with dg3d_set_variant. With the VALU contraction the results do not

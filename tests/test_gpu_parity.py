@@ -220,7 +220,7 @@ def test_kernel_variants_bitwise_identical(precision, scheme):
     p = small_c2(n_e=5, order=4, steps=4)
     dt = problem.cfl_dt_host(p)
     ref = None
-    for v in range(4):
+    for v in range(8):
         for x in range(4):
             ctx = problem.make_context(p, precision=precision)
             ctx.set_variant(scheme, v, x)
@@ -232,7 +232,7 @@ def test_kernel_variants_bitwise_identical(precision, scheme):
             assert np.array_equal(st, ref), (v, x)
     ctx = problem.make_context(p, precision=precision)
     v, x = ctx.autotune(scheme)
-    assert 0 <= v < 4 and 0 <= x < 4
+    assert 0 <= v < 8 and 0 <= x < 4
 
 
 @pytest.mark.parametrize("contraction", [dg3d.CONTRACT_MFMA_F32, dg3d.CONTRACT_MFMA_BF16])
@@ -268,7 +268,7 @@ def test_matrix_core_variants(contraction, scheme):
     p = small_c2(n_e=5, order=4, steps=4)
     dt = problem.cfl_dt_host(p)
     ref = None
-    for v in range(4):
+    for v in range(8):
         for x in range(4):
             runs = []
             for _ in range(2 if x == 0 else 1):
