@@ -60,8 +60,8 @@ struct dg3d_batch {
   // per order present (descending n): task lists, grids and the tuned shape
   struct Group {
     int n = 0;
-    dg3d::BatchTask* d_tasks[dg3d::kNumBatchVariants] = {};
-    int ntasks[dg3d::kNumBatchVariants] = {};
+    dg3d::BatchTask* d_tasks[2] = {nullptr, nullptr};  // per elements-per-workgroup shape
+    int ntasks[2] = {0, 0};
     int grid[dg3d::kNumBatchVariants] = {};
     int variant = -1;
     int64_t nodes = 0;
@@ -127,8 +127,8 @@ dg3d::BatchLaunch stage_launch(const dg3d_batch* b, const dg3d_batch::Group& g, 
                                int from) {
   dg3d::BatchLaunch B{};
   B.records = b->d_records;
-  B.tasks = g.d_tasks[variant];
-  B.ntasks = g.ntasks[variant];
+  B.tasks = g.d_tasks[variant & 1];
+  B.ntasks = g.ntasks[variant & 1];
   B.record = (halo && stage == 0) ? 10 + from : 2 * stage + from;
   B.mixed = (b->precision == DG3D_MIXED) ? 1 : 0;
   return B;
@@ -181,31 +181,34 @@ hipError_t build_graphs(dg3d_batch* b) {
   return hipSuccess;
 }
 
-// Per order, times every variant (shape x persistent or one task per workgroup) on a later-stage launch (reads and writes
+// Per order, times every variant (shape x persistent or one task per workgroup x early or late
+// RK-register loads) on a later-stage launch (reads and writes
 // du) that writes only arena 1 - cur and du; du is cleared first, and the run's next stage 0
 // overwrites both.
 int batch_autotune(dg3d_batch* b) {
   hipError_t e = hipMemsetAsync(b->du, 0, (b->precision == DG3D_MIXED ? 4 : 8) * (size_t)b->total_slots, b->stream);
   if (e != hipSuccess) return b->hip_fail(e, "batch autotune");
   for (auto& g : b->groups) {
-    float best = 1e30f;
-    int bv = 0;
-    for (int v = 0; v < dg3d::kNumBatchVariants; ++v) {
-      const dg3d::BatchLaunch B = stage_launch(b, g, v, 1, false, b->cur);
-      e = dg3d::launch_rhs_batch(B, g.n, v, g.grid[v], b->stream);
-      if (e == hipSuccess) e = hipEventRecord(b->ev0, b->stream);
-      for (int r = 0; r < 3 && e == hipSuccess; ++r) e = dg3d::launch_rhs_batch(B, g.n, v, g.grid[v], b->stream);
-      if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
-      if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
-      if (e != hipSuccess) return b->hip_fail(e, "batch autotune");
-      float ms = 0.f;
-      hipEventElapsedTime(&ms, b->ev0, b->ev1);
-      if (ms < best) {
-        best = ms;
-        bv = v;
+    // warm-up, then two sweeps of which each variant keeps the faster
+    const dg3d::BatchLaunch W = stage_launch(b, g, 0, 1, false, b->cur);
+    for (int r = 0; r < 4 && e == hipSuccess; ++r) e = dg3d::launch_rhs_batch(W, g.n, 0, g.grid[0], b->stream);
+    float best[dg3d::kNumBatchVariants];
+    for (float& x : best) x = 1e30f;
+    for (int sweep = 0; sweep < 2; ++sweep) {
+      for (int v = 0; v < dg3d::kNumBatchVariants; ++v) {
+        const dg3d::BatchLaunch B = stage_launch(b, g, v, 1, false, b->cur);
+        if (e == hipSuccess) e = dg3d::launch_rhs_batch(B, g.n, v, g.grid[v], b->stream);
+        if (e == hipSuccess) e = hipEventRecord(b->ev0, b->stream);
+        for (int r = 0; r < 3 && e == hipSuccess; ++r) e = dg3d::launch_rhs_batch(B, g.n, v, g.grid[v], b->stream);
+        if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
+        if (e != hipSuccess) return b->hip_fail(e, "batch autotune");
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, b->ev0, b->ev1);
+        best[v] = std::min(best[v], ms);
       }
     }
-    g.variant = bv;
+    g.variant = (int)(std::min_element(best, best + dg3d::kNumBatchVariants) - best);
   }
   b->tuned = true;
   return DG3D_OK;
@@ -402,7 +405,7 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
     g.n = n;
     for (int i = 0; i < count; ++i)
       if (b->solves[i].n == n) g.nodes += b->solves[i].E * b->solves[i].np;
-    for (int v = 0; v < dg3d::kNumBatchVariants; ++v) {
+    for (int v = 0; v < 2; ++v) {
       const int epb = dg3d::batch_epb(v, n);
       std::vector<dg3d::BatchTask> tasks;
       for (int i = 0; i < count; ++i) {
@@ -416,8 +419,10 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
       }
       hipMemcpy(g.d_tasks[v], tasks.data(), sizeof(dg3d::BatchTask) * tasks.size(), hipMemcpyHostToDevice);
       g.ntasks[v] = (int)tasks.size();
+    }
+    for (int v = 0; v < dg3d::kNumBatchVariants; ++v) {
       const int resident = dg3d::batch_max_grid(n, v, precision == DG3D_MIXED ? 1 : 0);
-      g.grid[v] = std::max(1, std::min(g.ntasks[v], resident > 0 ? resident : 1024));
+      g.grid[v] = std::max(1, std::min(g.ntasks[v & 1], resident > 0 ? resident : 1024));
     }
     g.variant = 0;
     e = hipStreamCreate(&g.stream);
@@ -519,7 +524,7 @@ int dg3d_batch_info(const dg3d_batch* b, int64_t* total_nodes, int* num_orders, 
     const int v = g.variant < 0 ? 0 : g.variant;
     if (orders) orders[i] = g.n - 1;
     if (variants) variants[i] = g.variant;
-    if (tasks) tasks[i] = g.ntasks[v];
+    if (tasks) tasks[i] = g.ntasks[v & 1];
     if (grids) grids[i] = g.grid[v];
   }
   return DG3D_OK;

@@ -128,8 +128,8 @@ struct BatchLaunch {
   int mixed;
 };
 // Per order, autotuned: bit 0 picks the elements-per-workgroup shape (the stage kernel's EPB0 or
-// EPB1), bit 1 one task per workgroup instead of the persistent loop.
-constexpr int kNumBatchVariants = 4;
+// EPB1), bit 1 one task per workgroup instead of the persistent loop, bit 2 late RK-register loads.
+constexpr int kNumBatchVariants = 8;
 int batch_epb(int variant, int n);
 // One persistent launch over the tasks of the solves of order n-1 (B.tasks holds only those).
 hipError_t launch_rhs_batch(const BatchLaunch& B, int n, int variant, int grid, hipStream_t s);

@@ -699,7 +699,7 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunc
 using ConstLaunchPtr = const __attribute__((address_space(4))) RhsLaunch*;
 using ConstTaskPtr = const __attribute__((address_space(4))) BatchTask*;
 
-template <int NN, int EPB, typename CT, typename DUT, bool PERSIST>
+template <int NN, int EPB, typename CT, typename DUT, bool PERSIST, bool LATE>
 __global__ void __launch_bounds__((Geo<NN, EPB>::NT)) k_rhs_batch(const BatchLaunch B) {
   __shared__ StageSmem<NN, EPB, CT, 0> sm;
   const ConstTaskPtr tasks = (ConstTaskPtr)B.tasks;
@@ -712,14 +712,15 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT)) k_rhs_batch(const BatchLau
       asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"((int)threadIdx.x));
       const int solve = tasks[t].solve, e0 = tasks[t].e0;
       const RhsLaunch& L = *(const RhsLaunch*)((ConstLaunchPtr)(B.records + (int64_t)solve * kBatchRecords + B.record));
-      stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI_LSRK, DUT, 0>(L, L.mesh, L.ext.trace, e0, tid, sm);
+      stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI_LSRK, DUT, 0, LATE>(L, L.mesh, L.ext.trace, e0, tid, sm);
     }
   } else {
     const int t = blockIdx.x;
     if (t >= B.ntasks) return;
     const int solve = tasks[t].solve, e0 = tasks[t].e0;
     const RhsLaunch& L = *(const RhsLaunch*)((ConstLaunchPtr)(B.records + (int64_t)solve * kBatchRecords + B.record));
-    stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI_LSRK, DUT, 0>(L, L.mesh, L.ext.trace, e0, (int)threadIdx.x, sm);
+    stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI_LSRK, DUT, 0, LATE>(L, L.mesh, L.ext.trace, e0, (int)threadIdx.x,
+                                                                     sm);
   }
 }
 
@@ -990,40 +991,50 @@ hipError_t launch_ldg_n(const LdgLaunch& L, hipStream_t s) {
 // Per-order launchers. The stage kernels of one order are instantiated in a translation unit of
 // their own (this file built with DG3D_TU_ORDER = 2..8), so the orders compile in parallel; the
 // file built without it holds the dispatch on the order and the order-independent kernels.
-template <int NN, int EPB, bool PERSIST>
+template <int NN, int EPB, bool PERSIST, bool LATE>
 hipError_t launch_batch_shape(const BatchLaunch& B, int grid, hipStream_t s) {
   constexpr int NT = Geo<NN, EPB>::NT;
   if (!PERSIST) grid = B.ntasks;
-  if (B.mixed) hipLaunchKernelGGL((k_rhs_batch<NN, EPB, float, float, PERSIST>), dim3(grid), dim3(NT), 0, s, B);
-  else hipLaunchKernelGGL((k_rhs_batch<NN, EPB, double, double, PERSIST>), dim3(grid), dim3(NT), 0, s, B);
+  if (B.mixed) hipLaunchKernelGGL((k_rhs_batch<NN, EPB, float, float, PERSIST, LATE>), dim3(grid), dim3(NT), 0, s, B);
+  else hipLaunchKernelGGL((k_rhs_batch<NN, EPB, double, double, PERSIST, LATE>), dim3(grid), dim3(NT), 0, s, B);
   return hipGetLastError();
 }
-// variant bit 0: elements-per-workgroup shape (EPB0 / EPB1); bit 1: one task per workgroup
+// variant bit 0: elements-per-workgroup shape (EPB0 / EPB1); bit 1: one task per workgroup;
+// bit 2: late RK-register loads
 template <int NN>
 hipError_t launch_batch_n(const BatchLaunch& B, int variant, int grid, hipStream_t s) {
+  using V = Variants<NN>;
   switch (variant) {
-    case 0: return launch_batch_shape<NN, Variants<NN>::EPB0, true>(B, grid, s);
-    case 1: return launch_batch_shape<NN, Variants<NN>::EPB1, true>(B, grid, s);
-    case 2: return launch_batch_shape<NN, Variants<NN>::EPB0, false>(B, grid, s);
-    default: return launch_batch_shape<NN, Variants<NN>::EPB1, false>(B, grid, s);
+    case 0: return launch_batch_shape<NN, V::EPB0, true, false>(B, grid, s);
+    case 1: return launch_batch_shape<NN, V::EPB1, true, false>(B, grid, s);
+    case 2: return launch_batch_shape<NN, V::EPB0, false, false>(B, grid, s);
+    case 3: return launch_batch_shape<NN, V::EPB1, false, false>(B, grid, s);
+    case 4: return launch_batch_shape<NN, V::EPB0, true, true>(B, grid, s);
+    case 5: return launch_batch_shape<NN, V::EPB1, true, true>(B, grid, s);
+    case 6: return launch_batch_shape<NN, V::EPB0, false, true>(B, grid, s);
+    default: return launch_batch_shape<NN, V::EPB1, false, true>(B, grid, s);
   }
 }
 
-template <int NN, int EPB>
+// resident workgroups of the persistent form of a variant (the grid of its launches)
+template <int NN, int EPB, bool LATE>
 int resident_shape(int mixed) {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   constexpr int NT = Geo<NN, EPB>::NT;
-  hipError_t e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, float, float, true>, NT, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, double, double, true>, NT, 0);
+  hipError_t e =
+      mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, float, float, true, LATE>, NT, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<NN, EPB, double, double, true, LATE>, NT, 0);
   if (e != hipSuccess || per < 1) per = 1;
   return cus * per;
 }
 template <int NN>
 int resident_n(int variant, int mixed) {
-  return (variant & 1) == 0 ? resident_shape<NN, Variants<NN>::EPB0>(mixed)
-                            : resident_shape<NN, Variants<NN>::EPB1>(mixed);
+  using V = Variants<NN>;
+  const bool alt = (variant & 1) != 0, late = (variant & 4) != 0;
+  if (late) return alt ? resident_shape<NN, V::EPB1, true>(mixed) : resident_shape<NN, V::EPB0, true>(mixed);
+  return alt ? resident_shape<NN, V::EPB1, false>(mixed) : resident_shape<NN, V::EPB0, false>(mixed);
 }
 
 }  // namespace

@@ -222,9 +222,10 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* solves, int precision, d
 const char* dg3d_batch_create_error(void);
 int dg3d_batch_destroy(dg3d_batch* batch);
 const char* dg3d_batch_last_error(const dg3d_batch* batch);
-/* Variant per order, 0..3: bit 0 the elements-per-workgroup shape (the stage kernel's two),
- * bit 1 one element group per workgroup instead of the persistent loop. dg3d_batch_autotune
- * writes one variant per order, in dg3d_batch_info's order. */
+/* Variant per order, 0..7: bit 0 the elements-per-workgroup shape (the stage kernel's two),
+ * bit 1 one element group per workgroup instead of the persistent loop, bit 2 RK-register loads
+ * after the first barrier. dg3d_batch_autotune writes one variant per order, in
+ * dg3d_batch_info's order. */
 int dg3d_batch_autotune(dg3d_batch* batch, int* variants);
 int dg3d_batch_set_variant(dg3d_batch* batch, int variant);
 /* nsteps LSRK54 steps of every solve, each with its own dt; synchronises. */

@@ -108,23 +108,23 @@ def test_batch_mixed_vs_fp64():
 
 @pytest.mark.parametrize("precision", [dg3d.FP64, dg3d.MIXED])
 def test_batch_shapes_bitwise_identical(precision):
-    """Every variant (two elements-per-workgroup shapes, persistent or one group per workgroup)
-    computes the same per-node arithmetic."""
+    """Every variant (two elements-per-workgroup shapes, persistent or one group per workgroup,
+    early or late RK-register loads) computes the same per-node arithmetic."""
     print(f"batch_shapes precision={precision}", flush=True)
     ps = mixed_set()
     states = []
-    for v in range(4):
+    for v in range(8):
         b = problem.make_batch(ps, precision=precision)
         b.set_variant(v)
         b.step(5)
         states.append([b.get_state(i) for i in range(len(ps))])
         b.close()
-    for v in range(1, 4):
+    for v in range(1, 8):
         for i in range(len(ps)):
             assert np.array_equal(states[0][i], states[v][i]), (v, i)
     b = problem.make_batch(ps, precision=precision)
     v = b.autotune()
-    assert len(v) == b.info()["num_orders"] and all(0 <= x < 4 for x in v)
+    assert len(v) == b.info()["num_orders"] and all(0 <= x < 8 for x in v)
     b.close()
 
 
