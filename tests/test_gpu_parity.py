@@ -283,3 +283,39 @@ def test_matrix_core_variants(contraction, scheme):
                 ref = runs[0]
             err = rel_l2(runs[0], ref)
             assert err.max() < 1e-6, (v, x, err)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("order", [1, 3, 4, 7])
+def test_operate_ldg(axis, order):
+    """Element::OperateQFromDRho (in place, Grid.cpp:667) then OperateDRhoFromQ into another
+    register, on heterogeneous D, anisotropic widths and random halo traces (with Q)."""
+    print(f"operate_ldg order={order} axis={axis}", flush=True)
+    p = heterogeneous(small_c2(n_e=3, order=order))
+    p.num_elements = (3, 4, 2)
+    p.widths = [np.full(3, 1e-3), np.linspace(0.8e-3, 1.2e-3, 4), np.full(2, 0.9e-3)]
+    E = 24
+    rng = np.random.default_rng(10 + order)
+    p.c0, p.rho0, p.b_over_a = 1500 + 100 * rng.random(E), 1000 + 200 * rng.random(E), 5.0 * rng.random(E)
+    p.sigma, p.diffusion = np.zeros(E), 1.0 + 4.0 * rng.random(E)
+    p.source = problem.Source()
+    ctx, o = _pair(p, nreg=2, with_q=True)
+    st = np.concatenate([rng.standard_normal(4 * p.dof) * np.repeat([1e-3, 1e-4, 1e-4, 1e-4], p.dof),
+                         1e-2 * rng.standard_normal(3 * p.dof)])
+    halos = random_halos(p)
+    _load_same(ctx, o, p, st, halos)
+    ctx.operate(dg3d.OP_QFROMDRHO, axis, 0, 0, 0)
+    o.operate(2, axis, 0, 0, 0)
+    g, r = ctx.get_state(0, nfields=7), o.get_state(0, nfields=7)
+    err = rel_l2(g, r, nfields=7)
+    assert np.linalg.norm(r.reshape(7, -1)[4 + axis]) > 0
+    assert err[4 + axis] < TOL_FP64, err
+    assert np.array_equal(g.reshape(7, -1)[:4], st.reshape(7, -1)[:4])  # rho', u, v, w untouched
+    ctx.zero(1)
+    o.zero(1)
+    ctx.operate(dg3d.OP_DRHOFROMQ, axis, 0, 0, 1)
+    o.operate(3, axis, 0, 0, 1)
+    g1, r1 = ctx.get_state(1, nfields=7), o.get_state(1, nfields=7)
+    err1 = rel_l2(g1, r1, nfields=7)
+    assert np.linalg.norm(r1.reshape(7, -1)[0]) > 0
+    assert err1[0] < TOL_FP64, err1

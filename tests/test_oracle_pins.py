@@ -244,3 +244,89 @@ def test_stability_cfl_definition_long_run():
     for _ in range(150):
         o.lsrk54(dt, 0, 1, 2)
     assert o.energy(0)[0] <= e0
+
+
+# ---- LDG diffusion (Element.cpp:623-758): exactness and the heat kernel ------------------
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_oracle_ldg_exact_for_polynomials(axis):
+    """Q_a = d rho'/dx_a and the DRhoFromQ increment D d^2 rho'/dx_a^2 (so that rho' follows the
+    heat equation), exactly for a cubic at N=3 through the element loops, with the exterior
+    continuing the polynomial."""
+    N = 3
+    p = _poly_problem(N)
+    D = 3.0
+    p.diffusion = np.full(p.E, D)
+    o = ob.Oracle(p, num_registers=2)
+    coords = [problem.axis_coords(p, a) for a in range(3)]
+    n = p.n
+    nE = p.num_elements
+    poly = np.polynomial.Polynomial([0.3, -1.0, 2.0, 0.7])
+    d1, d2 = poly.deriv(), poly.deriv(2)
+    X = [coords[0][:, None, None, :, None, None], coords[1][None, :, None, None, :, None],
+         coords[2][None, None, :, None, None, :]]
+    shape = tuple(nE) + (n, n, n)
+    xa = np.broadcast_to(X[axis], shape)
+    st = np.zeros((7,) + shape)
+    st[0] = 1e-4 * poly(xa)
+    o.set_state(0, st.reshape(-1))
+    ecs = problem.element_corners(p.corner, p.widths)
+    for face in range(6):
+        a = problem.FACE_AXIS[face]
+        b_ax, c_ax = (1, 2) if a == 0 else ((0, 2) if a == 1 else (0, 1))
+        tr = np.zeros((nE[b_ax], nE[c_ax], n, n, 3))
+        if a == axis:
+            xf = (ecs[a][-1] + p.widths[a][-1]) if face % 2 else ecs[a][0]
+            tr[..., 0] = 1e-4 * poly(xf)
+            tr[..., 2] = 1e-4 * d1(xf)    # the exterior Q_axis
+        else:
+            cb = coords[axis]
+            arr = cb[:, None, :, None] if axis == b_ax else cb[None, :, None, :]
+            tr[..., 0] = 1e-4 * poly(np.broadcast_to(arr, (nE[b_ax], nE[c_ax], n, n)))
+        o.set_halo(0, face, tr.reshape(-1))
+    o.operate(2, axis, 0, 0, 0)                      # QFromDRho, in place
+    q = o.get_state(0, nfields=7).reshape((7,) + shape)
+    expect_q = 1e-4 * d1(xa)
+    assert np.abs(q[4 + axis] - expect_q).max() < 1e-10 * np.abs(expect_q).max()
+    o.zero(1)
+    o.operate(3, axis, 0, 0, 1)                      # DRhoFromQ into register 1
+    out = o.get_state(1, nfields=7).reshape((7,) + shape)
+    expect = D * 1e-4 * d2(xa)
+    assert np.abs(out[0] - expect).max() < 1e-9 * np.abs(expect).max()
+
+
+def _heat_decay_error(N, ne=4, D=2.0e-3, T=5.0, steps=200):
+    nE = (ne, ne, ne)
+    L = 1.0
+    p = problem._base("heat", N, nE, (0, 0, 0), problem._uniform_widths(nE, (L / ne,) * 3), 1500.0, 1000.0, 0.0)
+    p.periodic = (1, 1, 1)
+    p.diffusion = np.full(p.E, D)
+    o = ob.Oracle(p, num_registers=3)
+    coords = problem.axis_coords(p, 1)
+    n = p.n
+    k = 2 * np.pi / L
+    shape = nE + (n, n, n)
+    y = np.broadcast_to(coords[None, :, None, None, :, None], shape)
+    st = np.zeros((7,) + shape)
+    st[0] = 1e-4 * np.sin(k * y)
+    o.set_state(0, st.reshape(-1))
+    dt = T / steps
+    o.set_time(0.0)
+    for _ in range(steps):
+        o.lsrk54(dt, 0, 1, 2, adv=0, diff=1, sponge=0)
+    got = o.get_state(0).reshape((4,) + shape)
+    decay = np.exp(-D * k * k * T)
+    expect = 1e-4 * np.sin(k * y) * decay
+    assert np.abs(got[1:]).max() == 0.0   # diffusion acts on rho' only
+    return np.abs(got[0] - expect).max() / 1e-4, decay
+
+
+def test_oracle_ldg_heat_kernel_decay():
+    """A Fourier mode of rho' in a periodic box under diffusion alone decays as
+    exp(-D k^2 t) (the Gaussian heat kernel's mode-by-mode form), through LSRK54; the error
+    falls spectrally with the order (about 10x per order here)."""
+    e4, decay = _heat_decay_error(4)
+    e6, _ = _heat_decay_error(6)
+    assert decay < 0.7          # a decay that a sign or scale error of D would miss
+    assert e6 < 5e-6, e6
+    assert e4 / e6 > 50, (e4, e6)
