@@ -52,6 +52,7 @@ struct dg3d_ctx {
   std::vector<double> h_mat;
   double* d_diff = nullptr;
   std::vector<double> h_diff;
+  bool has_diff = false;  // some element has D != 0: the fused stages add the LDG term
   double* d_opd[3] = {nullptr, nullptr, nullptr};
   float* d_opf[3] = {nullptr, nullptr, nullptr};
   std::vector<double> nodes, weights;
@@ -176,7 +177,9 @@ int upload_material(dg3d_ctx* c) {
     r[dg3d::MAT_BA] = c->h_mat[i * 4 + 2];
     r[dg3d::MAT_SIGMA] = c->h_mat[i * 4 + 3];
     r[dg3d::MAT_IRHO0] = 1.0 / c->h_mat[i * 4 + 1];
+    r[dg3d::MAT_DIFF] = c->h_diff[i];
   }
+  c->has_diff = std::any_of(c->h_diff.begin(), c->h_diff.end(), [](double d) { return d != 0.0; });
   hipError_t e = hipMemcpyAsync(c->d_mat, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice,
                                 c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "material upload");
@@ -197,8 +200,8 @@ int ensure_work(dg3d_ctx* c, int scheme) {
       e = hipMalloc(&c->work_du, bytes);
     }
   } else {
-    if (!c->work_k1) e = dmalloc(&c->work_k1, (size_t)4 * c->fs);
-    if (e == hipSuccess && !c->work_k2) e = dmalloc(&c->work_k2, (size_t)4 * c->fs);
+    if (!c->work_k1) e = dmalloc(&c->work_k1, (size_t)c->nfields * c->fs);
+    if (e == hipSuccess && !c->work_k2) e = dmalloc(&c->work_k2, (size_t)c->nfields * c->fs);
   }
   if (e != hipSuccess) return c->fail(DG3D_E_NOMEM, std::string("work register allocation: ") + hipGetErrorString(e));
   return DG3D_OK;
@@ -251,6 +254,20 @@ int run_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t, double** 
       e = c->ext_rhs(-1, t + 0.5 * dt, &L.ext);
     }
     if (e != hipSuccess) return c->hip_fail(e, "source trace");
+  }
+  if (c->has_diff && c->nfields == 7) {
+    // Grid.cpp:655-667: Q of the stage input from its rho' (in place), then the stage adds
+    // OperateDRhoFromQ for the three axes
+    dg3d::LdgLaunch Q{};
+    Q.n = c->n;
+    Q.mesh = c->mesh();
+    Q.ext = L.ext;
+    for (int i = 0; i < 3; ++i) Q.opd[i] = c->d_opd[i];
+    Q.in = L.q_in;
+    Q.out = const_cast<double*>(L.q_in);
+    hipError_t e = dg3d::launch_ldg_q(Q, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "LDG kernel launch");
+    L.ldg = 1;
   }
   hipError_t e = dg3d::launch_rhs(L, c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "stage kernel launch");
@@ -702,6 +719,8 @@ int dg3d_step(dg3d_ctx* c, int scheme, double dt, int nsteps, double* t) {
   if (scheme != DG3D_LSRK54 && scheme != DG3D_RK33) return c->fail(DG3D_E_INVALID, "step: unknown scheme");
   if (nsteps < 0) return c->fail(DG3D_E_INVALID, "step: nsteps < 0");
   if (!(dt > 0.0) || !std::isfinite(dt)) return c->fail(DG3D_E_INVALID, "step: dt must be positive");
+  if (c->has_diff && c->nfields < 7)
+    return c->fail(DG3D_E_INVALID, "step: diffusion (D != 0) needs a context created with with_q");
   int rc = ensure_work(c, scheme);
   if (rc) return rc;
   if (nsteps == 0) return DG3D_OK;

@@ -56,7 +56,7 @@ constexpr int elem_stride(int n) {
 inline int64_t field_slots(int64_t E, int n) { return (E * elem_stride(n) + 31) / 32 * 32; }
 
 constexpr int kMatStride = 8;  // per-element material record (doubles)
-enum MatSlot { MAT_C = 0, MAT_RHO0 = 1, MAT_BA = 2, MAT_SIGMA = 3, MAT_IRHO0 = 4 };
+enum MatSlot { MAT_C = 0, MAT_RHO0 = 1, MAT_BA = 2, MAT_SIGMA = 3, MAT_IRHO0 = 4, MAT_DIFF = 5 };
 
 struct MeshArgs {
   int nE[3];
@@ -68,7 +68,7 @@ struct MeshArgs {
   Div32 div12;        // divide by nE1*nE2
   Div32 div2;         // divide by nE2
   const double* inv_w[3];  // 1/width per element index along each axis
-  const double* mat;       // [E][kMatStride] = c0, rho0, B/A, sigma, 1/rho0, -, -, -
+  const double* mat;       // [E][kMatStride] = c0, rho0, B/A, sigma, 1/rho0, D, -, -
 };
 
 struct ExtArgs {
@@ -94,6 +94,7 @@ struct RhsLaunch {
   int xcd_remap;    // workgroup order 0..kNumOrders-1 (autotuned)
   int variant;      // kernel variant 0..kNumVariants-1 (autotuned)
   int contraction;  // mixed path: 0 VALU, 1 MFMA FP32, 2 MFMA split-BF16
+  int ldg;          // 1: add the LDG diffusion term from the Q fields of q_in (fields 4..6)
   MeshArgs mesh;
   ExtArgs ext;
   const double* opd[3];  // A, L0, LN in FP64
@@ -153,6 +154,9 @@ struct LdgLaunch {
   double* out;         // register written
 };
 hipError_t launch_ldg(const LdgLaunch& L, hipStream_t s);
+// Element::OperateQFromDRho for the three axes in one pass (Grid.cpp:655-667): Q_a of `in`
+// (fields 4..6) from rho' of `in` and the exterior traces, in place. FP64.
+hipError_t launch_ldg_q(const LdgLaunch& L, hipStream_t s);
 
 // out = out*a + in*b over nfields fields of length fs.
 hipError_t launch_axpby(double* out, double a, const double* in, double b, int64_t count, hipStream_t s);

@@ -376,7 +376,7 @@ __device__ __forceinline__ int face_neighbour(const MeshArgs& m, int he, const i
 }
 
 // LDS of one stage workgroup: EPB elements of order NN-1.
-template <int NN, int EPB, typename CT, int CONTRACT>
+template <int NN, int EPB, typename CT, int CONTRACT, bool LDG = false>
 struct StageSmem {
   static constexpr int NP = NN * NN * NN, NF2 = NN * NN;
   CT sF[EPB * 4 * NP];                    // P/rho0, m_x, m_y, m_z per node
@@ -384,6 +384,8 @@ struct StageSmem {
   CT sK[EPB * 6 * kNK];                   // flux coefficients per face
   CT sIw[EPB * 3];                        // 1/width per element and axis
   float sOut[CONTRACT ? EPB * 6 * NP : 1];  // matrix-core contraction results
+  CT sQ[LDG ? EPB * 3 * NP : 1];          // LDG: Q_x, Q_y, Q_z per node
+  CT sFq[LDG ? EPB * 6 * NF2 : 1];        // LDG: central trace of Q_axis per face node
   CT sA[NN * NN];
   CT sL0[NN];
   CT sLN[NN];
@@ -393,10 +395,14 @@ struct StageSmem {
 // arrays in L; traces: the 6 exterior face traces in memory, or null for none). NT is the
 // workgroup size (>= EPB * ES); every lane must call it, since it holds barriers. LATE issues the
 // RK-register loads (du / old, needed only by the epilogue) after the first barrier instead of
-// with the rest: fewer registers live at the peak, hence more waves, against later loads.
-template <int NN, int EPB, int NT, typename CT, int EPI, typename DUT, int CONTRACT, bool LATE = false>
+// with the rest: fewer registers live at the peak, hence more waves, against later loads. LDG
+// adds Element::OperateDRhoFromQ for the three axes (Element.cpp:691-758) from the Q fields of
+// q_in, which k_ldg_q filled for this stage: rho' += -D (1/w_a) (A Q_a + L0 Fq_prev + LN Fq_next),
+// with the central traces Fq_prev = (Q_o + Q_s)/2 and Fq_next = -(Q_s + Q_o)/2.
+template <int NN, int EPB, int NT, typename CT, int EPI, typename DUT, int CONTRACT, bool LATE = false,
+          bool LDG = false>
 __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m, const double* const* traces,
-                                           const int e0, const int tid, StageSmem<NN, EPB, CT, CONTRACT>& sm) {
+                                           const int e0, const int tid, StageSmem<NN, EPB, CT, CONTRACT, LDG>& sm) {
   using S = Geo<NN, EPB>;
   constexpr int NP = S::NP, NF2 = S::NF2;
   static_assert(NT >= EPB * S::ES, "workgroup too small for the element group");
@@ -458,16 +464,28 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   const CT c = (CT)bload<double>(rm, mi + MAT_C), r0 = (CT)bload<double>(rm, mi + MAT_RHO0);
   const CT ba = (CT)bload<double>(rm, mi + MAT_BA), sg = (CT)bload<double>(rm, mi + MAT_SIGMA);
   const CT ir0 = (CT)bload<double>(rm, mi + MAT_IRHO0);
+  Rsrc rqq[3];
+  double qq[3] = {0.0, 0.0, 0.0};
+  CT dcoef = CT(0);
+  if constexpr (LDG) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      rqq[a] = make_rsrc(L.q_in + (4 + a) * fs, fb64);
+      qq[a] = bload<double>(rqq[a], gi);
+    }
+    dcoef = (CT)bload<double>(rm, mi + MAT_DIFF);
+  }
 
   // face items (element, direction, face node) of this lane; item = fle | dir << 6 | fn << 9 |
   // self node << 16, or ~0u when the item has nothing to do
-  double nrho[R], nu[R];
+  double nrho[R], nu[R], nq[R];
   unsigned item[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int f = tid + r * NT;
     nrho[r] = 0.0;
     nu[r] = 0.0;
+    nq[r] = 0.0;
     item[r] = ~0u;
     if (f < NITEMS) {
       const int fle = f / (6 * NF2);
@@ -476,7 +494,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
       const int fn = rr - dir * NF2;
       const int axis = 2 - (dir >> 1);
       const int he = e0 + fle;
-      if (he < E && ((mask >> (2 * axis)) & 3u) != 0u) {
+      if (he < E && (LDG || ((mask >> (2 * axis)) & 3u) != 0u)) {
         const int b = fn / NN, cc = fn - (fn / NN) * NN;
         const int sn = face_node<NN>(axis, (dir & 1) ? NN - 1 : 0, b, cc);
         const int on = face_node<NN>(axis, (dir & 1) ? 0 : NN - 1, b, cc);
@@ -490,12 +508,18 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
           if (axis == 0) nu[r] = bload<double>(rq[1], oi);
           else if (axis == 1) nu[r] = bload<double>(rq[2], oi);
           else nu[r] = bload<double>(rq[3], oi);
+          if constexpr (LDG) {
+            if (axis == 0) nq[r] = bload<double>(rqq[0], oi);
+            else if (axis == 1) nq[r] = bload<double>(rqq[1], oi);
+            else nq[r] = bload<double>(rqq[2], oi);
+          }
         } else {
           const double* tr = traces ? traces[dir] : nullptr;
           if (tr) {
             const int64_t ti = ((int64_t)(-1 - nb) * NF2 + fn) * 3;
             nrho[r] = tr[ti];
             nu[r] = tr[ti + 1];
+            if constexpr (LDG) nq[r] = tr[ti + 2];
           }
         }
       }
@@ -547,6 +571,12 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
 #pragma unroll
     for (int a = 0; a < 3; ++a) F[(1 + a) * NP + node] = mass_f((CT)q[1 + a], rho, r0, ba);
   }
+  if constexpr (LDG) {
+    if (valid) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) sm.sQ[(le * 3 + a) * NP + node] = (CT)qq[a];
+    }
+  }
   if (helper) {
     CT K[kNK];
 #pragma unroll
@@ -585,6 +615,10 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     CT* Fl = sFl + ((fle * 6 + dir) * NF2 + fn) * 2;
     Fl[0] = K[0] * mom_o + K[1] * mom_s + K[2] * mass_o + K[3] * mass_s;
     Fl[1] = K[4] * mass_o + K[5] * mass_s + K[6] * mom_o + K[7] * mom_s;
+    if constexpr (LDG) {
+      const CT qs = sm.sQ[(fle * 3 + axis) * NP + sn], qo = (CT)nq[r];
+      sm.sFq[(fle * 6 + dir) * NF2 + fn] = (dir & 1) ? CT(-0.5) * (qs + qo) : CT(0.5) * (qo + qs);
+    }
   }
   __syncthreads();
   if constexpr (CONTRACT != 0) {
@@ -628,6 +662,24 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     const CT* fhi = Fl + ((dlo + 1) * NF2 + fn) * 2;
     if (am & 1u) du[a] = iw * (dp - (flo[0] * l0 + fhi[0] * ln));
     if (am & 2u) drho += iw * (dm - (flo[1] * l0 + fhi[1] * ln));
+  }
+  if constexpr (LDG) {
+    if (valid) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int ix = idx[a];
+        const int ls = (a == 0) ? NF2 : (a == 1 ? NN : 1);
+        const int base = node - ix * ls;
+        const int fn = (a == 0) ? idx[1] * NN + idx[2] : (a == 1 ? idx[0] * NN + idx[2] : idx[0] * NN + idx[1]);
+        const int dlo = (a == 0) ? 4 : (a == 1 ? 2 : 0);
+        const CT* Qa = sm.sQ + (le * 3 + a) * NP;
+        CT sq = CT(0);
+#pragma unroll
+        for (int mm = 0; mm < NN; ++mm) sq += sA[ix * NN + mm] * Qa[base + mm * ls];
+        sq += sm.sFq[(le * 6 + dlo) * NF2 + fn] * sL0[ix] + sm.sFq[(le * 6 + dlo + 1) * NF2 + fn] * sLN[ix];
+        drho += -dcoef * (sq * sIw[le * 3 + a]);
+      }
+    }
   }
   if (mask & kTermSponge) {
 #pragma unroll
@@ -677,17 +729,18 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   }
 }
 
-template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT, int CONTRACT = 0, bool LATE = false>
+template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT, int CONTRACT = 0, bool LATE = false,
+          bool LDG = false>
 __global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunch L) {
-  __shared__ StageSmem<NN, EPB, CT, CONTRACT> sm;
+  __shared__ StageSmem<NN, EPB, CT, CONTRACT, LDG> sm;
   const MeshArgs& m = L.mesh;
   int64_t grp;
   if (L.xcd_remap == 3)
     grp = tiled_group(remap_block(blockIdx.x, gridDim.x, 1, 1), m.nE[0], m.nE[1], m.nE[2], EPB);
   else
     grp = remap_block(blockIdx.x, gridDim.x, L.xcd_remap, order_run(m.nE[2], EPB));
-  stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI, DUT, CONTRACT, LATE>(L, m, L.ext.trace, (int)(grp * EPB),
-                                                                      (int)threadIdx.x, sm);
+  stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI, DUT, CONTRACT, LATE, LDG>(L, m, L.ext.trace, (int)(grp * EPB),
+                                                                           (int)threadIdx.x, sm);
 }
 
 // ---- batched persistent stage kernels ---------------------------------------------------
@@ -823,6 +876,69 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
   }
 }
 
+// Element::OperateQFromDRho (Element.cpp:623-690) for the three axes in one pass, in place:
+// Q_a = -(1/w_a) (A rho' + L0 F_prev + LN F_next), with the central traces F_prev =
+// (rho'_o + rho'_s)/2 and F_next = -(rho'_s + rho'_o)/2 (the exterior trace on block faces,
+// the source included). FP64, one lane per node; writes fields 4..6, reads field 0.
+template <int NN>
+__global__ void __launch_bounds__(Shape<NN>::NT) k_ldg_q(const LdgLaunch L) {
+  using S = Shape<NN>;
+  constexpr int NP = S::NP, NF2 = S::NF2, EPB = S::EPB, ES = S::ES;
+  __shared__ double sV[EPB * NP];
+  __shared__ double sFl[EPB * 6 * NF2];
+  const MeshArgs& m = L.mesh;
+  const int tid = threadIdx.x;
+  const int e0 = (int)blockIdx.x * EPB;
+  const int le = tid / NP;
+  const int node = tid - le * NP;
+  const int e = e0 + le;
+  const bool valid = (le < EPB) && (e < (int)m.E);
+  const int64_t gi = (int64_t)e * ES + node;
+  if (valid) sV[le * NP + node] = L.in[gi];
+  __syncthreads();
+  for (int f = tid; f < EPB * 6 * NF2; f += S::NT) {
+    const int fle = f / (6 * NF2);
+    const int r = f - fle * 6 * NF2;
+    const int dir = r / NF2;
+    const int fn = r - dir * NF2;
+    const int fe = e0 + fle;
+    if (fe >= (int)m.E) continue;
+    const int axis = 2 - (dir >> 1), side = dir & 1;
+    const int b = fn / NN, cc = fn - (fn / NN) * NN;
+    const int sn = face_node<NN>(axis, side ? NN - 1 : 0, b, cc);
+    const int on = face_node<NN>(axis, side ? 0 : NN - 1, b, cc);
+    const double vs = sV[fle * NP + sn];
+    int ec[3];
+    elem_coords(m, (uint32_t)fe, ec);
+    const int nb = face_neighbour(m, fe, ec, dir);
+    double vo = 0.0;
+    if (nb >= 0) {
+      vo = L.in[(int64_t)nb * ES + on];
+    } else {
+      const double* tr = L.ext.trace[dir];
+      if (tr) vo = tr[((int64_t)(-1 - nb) * NF2 + fn) * 3];
+    }
+    sFl[(fle * 6 + dir) * NF2 + fn] = side ? -0.5 * (vs + vo) : 0.5 * (vo + vs);
+  }
+  __syncthreads();
+  if (!valid) return;
+  const int idx[3] = {node / NF2, (node / NN) % NN, node % NN};
+  int ec[3];
+  elem_coords(m, (uint32_t)e, ec);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int ix = idx[a];
+    const int ls = (a == 0) ? NF2 : (a == 1 ? NN : 1);
+    const int base = node - ix * ls;
+    const int fn = (a == 0) ? idx[1] * NN + idx[2] : (a == 1 ? idx[0] * NN + idx[2] : idx[0] * NN + idx[1]);
+    const int dlo = (a == 0) ? 4 : (a == 1 ? 2 : 0);
+    double sv = 0.0;
+    for (int mm = 0; mm < NN; ++mm) sv += L.opd[0][ix * NN + mm] * sV[le * NP + base + mm * ls];
+    sv += sFl[(le * 6 + dlo) * NF2 + fn] * L.opd[1][ix] + sFl[(le * 6 + dlo + 1) * NF2 + fn] * L.opd[2][ix];
+    L.out[(4 + a) * m.fs + gi] = -(sv * m.inv_w[a][ec[a]]);
+  }
+}
+
 #ifndef DG3D_TU_ORDER
 __global__ void k_axpby(double* out, double a, const double* in, double b, int64_t count) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
@@ -936,6 +1052,26 @@ __global__ void k_probe(const double* reg, MeshArgs m, int n, const int64_t* nod
 }
 #endif  // DG3D_TU_ORDER
 
+// The fused stage with the LDG diffusion term: default shape, VALU contraction.
+template <int NN>
+hipError_t launch_rhs_ldg(const RhsLaunch& L, hipStream_t s) {
+  constexpr int EPB = Variants<NN>::EPB0;
+  using S = Geo<NN, EPB>;
+  const int64_t blocks = (L.mesh.E + EPB - 1) / EPB;
+  if (blocks <= 0) return hipSuccess;
+  dim3 grid((unsigned)blocks), block(S::NT);
+  if (L.epilogue == EPI_LSRK) {
+    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, 1, double, EPI_LSRK, double, 0, false, true>), grid, block, 0, s, L);
+    else hipLaunchKernelGGL((k_rhs<NN, EPB, 1, float, EPI_LSRK, float, 0, false, true>), grid, block, 0, s, L);
+  } else if (L.epilogue == EPI_RK33) {
+    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, 1, double, EPI_RK33, double, 0, false, true>), grid, block, 0, s, L);
+    else hipLaunchKernelGGL((k_rhs<NN, EPB, 1, float, EPI_RK33, double, 0, false, true>), grid, block, 0, s, L);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <int NN, int EPB, int MINW, bool LATE>
 hipError_t launch_rhs_v(const RhsLaunch& L, hipStream_t s) {
   using S = Geo<NN, EPB>;
@@ -964,6 +1100,7 @@ hipError_t launch_rhs_v(const RhsLaunch& L, hipStream_t s) {
 template <int NN>
 hipError_t launch_rhs_n(const RhsLaunch& L, hipStream_t s) {
   using V = Variants<NN>;
+  if (L.ldg) return launch_rhs_ldg<NN>(L, s);
   // the accumulate epilogue (Block::Operate*) only uses the default shape
   const int v = (L.epilogue == EPI_ACCUM) ? 0 : L.variant;
   switch (v) {
@@ -977,6 +1114,15 @@ hipError_t launch_rhs_n(const RhsLaunch& L, hipStream_t s) {
     case 7: return launch_rhs_v<NN, V::EPB1, V::MINW1, true>(L, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+template <int NN>
+hipError_t launch_ldg_q_n(const LdgLaunch& L, hipStream_t s) {
+  using S = Shape<NN>;
+  const int64_t blocks = (L.mesh.E + S::EPB - 1) / S::EPB;
+  if (blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_ldg_q<NN>), dim3((unsigned)blocks), dim3(S::NT), 0, s, L);
+  return hipGetLastError();
 }
 
 template <int NN>
@@ -1042,6 +1188,7 @@ int resident_n(int variant, int mixed) {
 #define DG3D_ORDER_API(n)                                                                     \
   hipError_t launch_rhs_o##n(const RhsLaunch& L, hipStream_t s);                             \
   hipError_t launch_ldg_o##n(const LdgLaunch& L, hipStream_t s);                             \
+  hipError_t launch_ldg_q_o##n(const LdgLaunch& L, hipStream_t s);                           \
   hipError_t launch_batch_o##n(const BatchLaunch& B, int variant, int grid, hipStream_t s);  \
   int resident_o##n(int variant, int mixed);
 DG3D_ORDER_API(2)
@@ -1061,6 +1208,9 @@ hipError_t DG3D_CAT(launch_rhs_o, DG3D_TU_ORDER)(const RhsLaunch& L, hipStream_t
 }
 hipError_t DG3D_CAT(launch_ldg_o, DG3D_TU_ORDER)(const LdgLaunch& L, hipStream_t s) {
   return launch_ldg_n<DG3D_TU_ORDER>(L, s);
+}
+hipError_t DG3D_CAT(launch_ldg_q_o, DG3D_TU_ORDER)(const LdgLaunch& L, hipStream_t s) {
+  return launch_ldg_q_n<DG3D_TU_ORDER>(L, s);
 }
 hipError_t DG3D_CAT(launch_batch_o, DG3D_TU_ORDER)(const BatchLaunch& B, int variant, int grid, hipStream_t s) {
   return launch_batch_n<DG3D_TU_ORDER>(B, variant, grid, s);
@@ -1090,6 +1240,19 @@ hipError_t launch_ldg(const LdgLaunch& L, hipStream_t s) {
     case 6: return launch_ldg_o6(L, s);
     case 7: return launch_ldg_o7(L, s);
     case 8: return launch_ldg_o8(L, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_ldg_q(const LdgLaunch& L, hipStream_t s) {
+  switch (L.n) {
+    case 2: return launch_ldg_q_o2(L, s);
+    case 3: return launch_ldg_q_o3(L, s);
+    case 4: return launch_ldg_q_o4(L, s);
+    case 5: return launch_ldg_q_o5(L, s);
+    case 6: return launch_ldg_q_o6(L, s);
+    case 7: return launch_ldg_q_o7(L, s);
+    case 8: return launch_ldg_q_o8(L, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -171,9 +171,9 @@ int main(int argc, char* argv[]) {
   }
   const unsigned NUM_TIMESTEPS = p.NUM_OUTSTEPS * p.NUM_TIME_SUBSTEPS;
   std::cout << "Creating mesh\n";
-  const bool fused = p.FUSED && !p.eval_diff && p.eval_advect;
+  const bool fused = p.FUSED && p.eval_advect;
   if (!fused && p.SCHEME == 1) {
-    std::cerr << "SCHEME = LSRK54 runs on the fused path only (no diffusion, advection on)\n";
+    std::cerr << "SCHEME = LSRK54 runs on the fused path only (advection on)\n";
     return ERRCODE_INVALID_PARAMETER;
   }
   int periodic[3] = {p.IS_PERIODIC[0], p.IS_PERIODIC[1], p.IS_PERIODIC[2]};

@@ -1,6 +1,7 @@
-"""End to end through the drop-in: parameters1.txt -> C++ driver (Grid::RK33 through the Block
-operators with LDG diffusion, 4 reference blocks) -> HDF5 files in the reference layout, checked
-against the oracle on the same run."""
+"""End to end through the drop-in: parameters1.txt (RK33 with LDG diffusion, 4 reference blocks)
+-> C++ driver -> HDF5 files in the reference layout, checked against the oracle on the same run.
+Both driver paths: the fused stage kernels (FUSED = 1, the default) and Grid::RK33 through the
+Block operators (FUSED = 0)."""
 import os
 import subprocess
 
@@ -64,9 +65,10 @@ def _gather(p, out_dir, step):
 
 
 @pytest.mark.skipif(not h5read.available(), reason="HDF5 library not available")
-def test_driver_parameters1_matches_oracle(tmp_path):
-    print("driver_parameters1_matches_oracle", flush=True)
-    pf = _params(tmp_path, "")
+@pytest.mark.parametrize("fused", [1, 0])
+def test_driver_parameters1_matches_oracle(tmp_path, fused):
+    print(f"driver_parameters1_matches_oracle fused={fused}", flush=True)
+    pf = _params(tmp_path, f"FUSED = {fused}\n")
     out = tmp_path / "out"
     r = subprocess.run([DRIVER, str(pf), str(out)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]

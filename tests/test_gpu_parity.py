@@ -319,3 +319,51 @@ def test_operate_ldg(axis, order):
     err1 = rel_l2(g1, r1, nfields=7)
     assert np.linalg.norm(r1.reshape(7, -1)[0]) > 0
     assert err1[0] < TOL_FP64, err1
+
+
+@pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
+@pytest.mark.parametrize("order", [2, 4])
+def test_fused_stage_with_ldg_diffusion(scheme, order):
+    """dg3d_step with diffusion on: per stage the Q kernel, then the fused stage with the
+    OperateDRhoFromQ term, against the oracle's Grid::RK33 order (QFromDRho, DRhoFromQ,
+    UFromV, sponge); heterogeneous D and material, B/A=5 burst source."""
+    print(f"fused_ldg scheme={scheme} order={order}", flush=True)
+    p = heterogeneous(small_c2(n_e=4, order=order, steps=40))
+    rng = np.random.default_rng(order)
+    p.diffusion = 1.0e-3 * (1.0 + rng.random(p.E))   # m^2/s at 0.75 mm elements: a visible term
+    ctx = problem.make_context(p, with_q=True, num_registers=1)
+    o = Oracle(p, literal=False, num_registers=4)
+    load_problem_into(o, p, 0)
+    dt = problem.cfl_dt_host(p)
+    ctx.step(scheme, dt, p.steps, 0.0)
+    o.set_time(0.0)
+    for _ in range(p.steps):
+        if scheme == dg3d.RK33:
+            o.rk33(dt, 3, 0, 1, 2, adv=1, diff=1, sponge=1)
+            o.copy(0, 3)
+        else:
+            o.lsrk54(dt, 0, 1, 2, adv=1, diff=1, sponge=1)
+    g, r = ctx.get_state(0), o.get_state(0)
+    err = rel_l2(g, r)
+    assert np.linalg.norm(r) > 0
+    assert err.max() < TOL_FP64, err
+    # the diffusion term is not negligible in this run
+    ctx2 = problem.make_context(p, with_q=True, num_registers=1)
+    ctx2.set_material(p.c0, p.rho0, p.b_over_a, p.sigma, np.zeros(p.E))
+    ctx2.step(scheme, dt, p.steps, 0.0)
+    assert rel_l2(ctx2.get_state(0), r).max() > 1e3 * TOL_FP64
+
+
+def test_fused_ldg_mixed_vs_fp64():
+    print("fused_ldg_mixed", flush=True)
+    p = heterogeneous(small_c2(n_e=4, order=4, steps=40))
+    p.diffusion = np.full(p.E, 1.0e-3)
+    dt = problem.cfl_dt_host(p)
+    st = []
+    for prec in (dg3d.FP64, dg3d.MIXED):
+        ctx = problem.make_context(p, precision=prec, with_q=True, num_registers=1)
+        ctx.step(dg3d.LSRK54, dt, p.steps, 0.0)
+        st.append(ctx.get_state(0))
+        ctx.close()
+    err = rel_l2(st[1], st[0])
+    assert 0.0 < err.max() < TOL_MIXED, err
