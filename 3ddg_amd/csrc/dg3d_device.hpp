@@ -76,7 +76,8 @@ struct ExtArgs {
   int src_face;            // -1 none
   double amp, freq, phase, t0, tau;
   const double* lead;      // [faceElem][n*n] or null
-  double t;                // stage time
+  double t;                // stage time; with `clock`, the offset of the stage from clock[0]
+  const double* clock;     // device copy of the step's start time, or null
 };
 
 template <typename CT>
@@ -136,8 +137,9 @@ int batch_epb(int variant, int n);
 hipError_t launch_rhs_batch(const BatchLaunch& B, int n, int variant, int grid, hipStream_t s);
 int batch_max_grid(int n, int variant, int mixed);  // resident workgroups on the device
 
-// Exterior trace of the source face at stage time x.t (halo trace + boundary source) into `out`
-// ([faceElem][n*n][3]); the RHS kernel then reads it as that face's halo.
+// Exterior trace of the source face at the stage time (x.t, or clock[0] + x.t) into `out`
+// (halo trace + boundary source), [faceElem][n*n][3]; the RHS kernel then reads it as that face's
+// halo.
 hipError_t launch_source_trace(const ExtArgs& x, const MeshArgs& m, int n, const double* halo, double* out,
                                hipStream_t s);
 
@@ -169,7 +171,12 @@ hipError_t launch_l2diff(const double* a, const double* b, int64_t fs, int nfiel
 hipError_t launch_energy(const double* reg, const MeshArgs& m, int n, const double* wq, double* partials,
                          hipStream_t s);
 hipError_t launch_cfl(const MeshArgs& m, double dxi_half, double* partials, hipStream_t s);
+// Probe values into row *step of `out` ([cap][count]); nothing once *step reaches cap.
 hipError_t launch_probe(const double* reg, const MeshArgs& m, int n, const int64_t* nodes, int count,
-                        double* out, hipStream_t s);
+                        double* out, const int* step, int cap, hipStream_t s);
+// Device clock of dg3d_step: clock[0] = t; the step counter of the probes is left alone.
+hipError_t launch_clock_set(double* clock, double t, hipStream_t s);
+// End of a step: clock[0] += dt, and the probe step counter advances while below cap.
+hipError_t launch_step_end(double* clock, double dt, int* probe_step, int cap, hipStream_t s);
 
 }  // namespace dg3d

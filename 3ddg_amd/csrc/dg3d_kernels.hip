@@ -796,7 +796,8 @@ __global__ void k_source_trace(ExtArgs x, MeshArgs m, int n, const double* halo,
   e3[axis == 0 ? 1 : 0] = eb;
   e3[axis == 2 ? 1 : 2] = ec;
   const int64_t e = ((int64_t)e3[0] * m.nE[1] + e3[1]) * m.nE[2] + e3[2];
-  const double t = x.t + (x.lead ? x.lead[i] : 0.0);
+  const double ts = x.clock ? x.clock[0] + x.t : x.t;
+  const double t = ts + (x.lead ? x.lead[i] : 0.0);
   const double sv = source_value(x, t);
   const double cd = m.mat[e * kMatStride + MAT_C], rd = m.mat[e * kMatStride + MAT_RHO0];
   const double h0 = halo ? halo[i * 3] : 0.0, h1 = halo ? halo[i * 3 + 1] : 0.0, h2 = halo ? halo[i * 3 + 2] : 0.0;
@@ -1040,15 +1041,28 @@ __global__ void __launch_bounds__(kRedThreads) k_cfl(MeshArgs m, double dxi_half
   if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
-__global__ void k_probe(const double* reg, MeshArgs m, int n, const int64_t* nodes, int count, double* out) {
+__global__ void k_probe(const double* reg, MeshArgs m, int n, const int64_t* nodes, int count, double* buf,
+                        const int* step, int cap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+  const int row = step[0];
+  if (i >= count || row >= cap) return;
+  double* out = buf + (int64_t)row * count;
   const int64_t g = nodes[i];  // device slot
   const int64_t e = g / m.es;
   const double c = m.mat[e * kMatStride + MAT_C], r0 = m.mat[e * kMatStride + MAT_RHO0];
   const double ba = m.mat[e * kMatStride + MAT_BA];
   const double r = reg[g];
   out[i] = c * c * (r + ba * 0.5 / r0 * r * r);
+}
+
+__global__ void k_clock_set(double* clock, double t) {
+  if (threadIdx.x == 0) clock[0] = t;
+}
+
+__global__ void k_step_end(double* clock, double dt, int* probe_step, int cap) {
+  if (threadIdx.x != 0) return;
+  clock[0] += dt;
+  if (probe_step[0] < cap) probe_step[0] += 1;
 }
 #endif  // DG3D_TU_ORDER
 
@@ -1343,9 +1357,19 @@ hipError_t launch_cfl(const MeshArgs& m, double dxi_half, double* part, hipStrea
   return hipGetLastError();
 }
 hipError_t launch_probe(const double* reg, const MeshArgs& m, int n, const int64_t* nodes, int count, double* out,
-                        hipStream_t s) {
+                        const int* step, int cap, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_probe, dim3((count + 63) / 64), dim3(64), 0, s, reg, m, n, nodes, count, out);
+  hipLaunchKernelGGL(k_probe, dim3((count + 63) / 64), dim3(64), 0, s, reg, m, n, nodes, count, out, step, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_clock_set(double* clock, double t, hipStream_t s) {
+  hipLaunchKernelGGL(k_clock_set, dim3(1), dim3(64), 0, s, clock, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_end(double* clock, double dt, int* probe_step, int cap, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, clock, dt, probe_step, cap);
   return hipGetLastError();
 }
 
