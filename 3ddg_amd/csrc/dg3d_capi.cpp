@@ -24,6 +24,19 @@ using dg3d::kLsC;
 
 thread_local std::string g_create_error;
 
+// One captured step of dg3d_step: the launches of the step (stages, probe, clock), keyed by the
+// registers it starts from and dt; everything else it reads is fixed until a setter drops it.
+struct StepGraph {
+  hipGraphExec_t exec = nullptr;
+  int scheme = -1;
+  double dt = 0.0;
+  double* cur = nullptr;    // state register the step reads
+  double* other = nullptr;  // LSRK54: the second state buffer
+  double* cur_end = nullptr;
+  double* other_end = nullptr;
+};
+constexpr int kMaxStepGraphs = 4;
+
 struct Register {
   double* d = nullptr;          // nfields * fs doubles
   double* halo[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -60,6 +73,10 @@ struct dg3d_ctx {
   dg3d_source src{};
   double* d_lead = nullptr;
   double* d_src_trace = nullptr;  // exterior trace of the source face at the current stage time
+  double* d_clock = nullptr;    // dg3d_step's time at the start of the current step
+  int* d_probe_step = nullptr;  // probe row the next step writes
+  bool use_graphs = true;
+  StepGraph graphs[kMaxStepGraphs];
   int64_t* d_probe_nodes = nullptr;
   double* d_probe_buf = nullptr;
   int probe_count = 0, probe_cap = 0, probe_steps = 0;
@@ -104,7 +121,7 @@ struct dg3d_ctx {
     return m;
   }
   dg3d::ExtArgs ext_for(int reg, double t) const {
-    dg3d::ExtArgs x;
+    dg3d::ExtArgs x{};
     for (int f = 0; f < 6; ++f) x.trace[f] = (reg >= 0 && regs[reg].halo_valid[f]) ? regs[reg].halo[f] : nullptr;
     x.src_face = src.face;
     x.amp = src.amplitude;
@@ -114,13 +131,16 @@ struct dg3d_ctx {
     x.tau = src.tau;
     x.lead = d_lead;
     x.t = t;
+    x.clock = nullptr;
     return x;
   }
   bool reg_ok(int r) const { return r >= 0 && r < (int)regs.size(); }
   // Exterior arguments of the RHS kernel: with a source, a small kernel first writes the source
   // face's trace (halo + source at time t) and the RHS kernel reads it as that face's halo.
-  hipError_t ext_rhs(int reg, double t, dg3d::ExtArgs* out) const {
+  // With dev_clock, t is the stage's offset from the device clock of dg3d_step.
+  hipError_t ext_rhs(int reg, double t, dg3d::ExtArgs* out, bool dev_clock = false) const {
     dg3d::ExtArgs x = ext_for(reg, t);
+    if (dev_clock) x.clock = d_clock;
     if (src.face >= 0) {
       hipError_t e = dg3d::launch_source_trace(x, mesh(), n, x.trace[src.face], d_src_trace, stream);
       if (e != hipSuccess) return e;
@@ -139,8 +159,16 @@ hipError_t dmalloc(T** p, size_t count) {
   return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (count ? count : 1));
 }
 
+void drop_graphs(dg3d_ctx* c) {
+  for (StepGraph& g : c->graphs) {
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    g = StepGraph{};
+  }
+}
+
 void free_ctx(dg3d_ctx* c) {
   if (!c) return;
+  drop_graphs(c);
   for (auto& r : c->regs) {
     hipFree(r.d);
     for (auto* h : r.halo) hipFree(h);
@@ -161,6 +189,8 @@ void free_ctx(dg3d_ctx* c) {
   hipFree(c->d_src_trace);
   hipFree(c->d_probe_nodes);
   hipFree(c->d_probe_buf);
+  hipFree(c->d_clock);
+  hipFree(c->d_probe_step);
   hipFree(c->d_part);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -221,7 +251,10 @@ dg3d::RhsLaunch base_launch(const dg3d_ctx* c) {
   return L;
 }
 
-int run_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t, double** cur, double** other, bool first) {
+// One fused stage. With dev_clock, t is 0 and the source trace reads the step's start time from
+// the device clock, so that the launches do not depend on the step being run.
+int run_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t, double** cur, double** other, bool first,
+              bool dev_clock = false) {
   dg3d::RhsLaunch L = base_launch(c);
   if (c->tune_variant[scheme] >= 0) {
     L.variant = c->tune_variant[scheme];
@@ -237,7 +270,7 @@ int run_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t, double** 
     L.du = c->work_du;
     L.a = kLsA[stage];
     L.b = kLsB[stage];
-    hipError_t e = c->ext_rhs(first ? 0 : -1, t + kLsC[stage] * dt, &L.ext);
+    hipError_t e = c->ext_rhs(first ? 0 : -1, t + kLsC[stage] * dt, &L.ext, dev_clock);
     if (e != hipSuccess) return c->hip_fail(e, "source trace");
   } else {
     L.epilogue = dg3d::EPI_RK33;
@@ -245,13 +278,13 @@ int run_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t, double** 
     hipError_t e;
     if (stage == 0) {
       L.q_in = u0; L.out = c->work_k1; L.old = nullptr; L.a = 0.0; L.b = 1.0;
-      e = c->ext_rhs(first ? 0 : -1, t, &L.ext);
+      e = c->ext_rhs(first ? 0 : -1, t, &L.ext, dev_clock);
     } else if (stage == 1) {
       L.q_in = c->work_k1; L.out = c->work_k2; L.old = u0; L.a = 0.75; L.b = 0.25;
-      e = c->ext_rhs(-1, t + dt, &L.ext);
+      e = c->ext_rhs(-1, t + dt, &L.ext, dev_clock);
     } else {
       L.q_in = c->work_k2; L.out = u0; L.old = u0; L.a = 1.0 / 3.0; L.b = 2.0 / 3.0;
-      e = c->ext_rhs(-1, t + 0.5 * dt, &L.ext);
+      e = c->ext_rhs(-1, t + 0.5 * dt, &L.ext, dev_clock);
     }
     if (e != hipSuccess) return c->hip_fail(e, "source trace");
   }
@@ -275,6 +308,65 @@ int run_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t, double** 
   return DG3D_OK;
 }
 
+// The launches of one step of dg3d_step: the stages, then the probe row and the device clock.
+// Only the first stage of a call may read register 0's halo traces (`first`).
+int launch_step(dg3d_ctx* c, int scheme, double dt, double** cur, double** other, bool first) {
+  const int ns = (scheme == DG3D_LSRK54) ? 5 : 3;
+  for (int st = 0; st < ns; ++st) {
+    const int rc = run_stage(c, scheme, st, dt, 0.0, cur, other, first && st == 0, true);
+    if (rc) return rc;
+  }
+  hipError_t e = hipSuccess;
+  if (c->probe_count > 0)
+    e = dg3d::launch_probe(*cur, c->mesh(), c->n, c->d_probe_nodes, c->probe_count, c->d_probe_buf, c->d_probe_step,
+                           c->probe_cap, c->stream);
+  if (e == hipSuccess && (c->src.face >= 0 || c->probe_count > 0))
+    e = dg3d::launch_step_end(c->d_clock, dt, c->d_probe_step, c->probe_cap, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "step");
+  return DG3D_OK;
+}
+
+// The captured form of launch_step for (scheme, dt, cur, other), captured on first use.
+int step_graph(dg3d_ctx* c, int scheme, double dt, double* cur, double* other, StepGraph** out) {
+  for (StepGraph& g : c->graphs)
+    if (g.exec && g.scheme == scheme && g.dt == dt && g.cur == cur && g.other == other) {
+      *out = &g;
+      return DG3D_OK;
+    }
+  StepGraph* slot = nullptr;
+  for (StepGraph& g : c->graphs)
+    if (!g.exec) { slot = &g; break; }
+  if (!slot) {
+    drop_graphs(c);
+    slot = &c->graphs[0];
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return c->hip_fail(e, "step graph capture");
+  double* a = cur;
+  double* b = other;
+  const int rc = launch_step(c, scheme, dt, &a, &b, false);
+  e = hipStreamEndCapture(c->stream, &graph);
+  if (rc || e != hipSuccess) {
+    if (graph) hipGraphDestroy(graph);
+    return rc ? rc : c->hip_fail(e, "step graph capture");
+  }
+  e = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    slot->exec = nullptr;
+    return c->hip_fail(e, "step graph instantiate");
+  }
+  slot->scheme = scheme;
+  slot->dt = dt;
+  slot->cur = cur;
+  slot->other = other;
+  slot->cur_end = a;
+  slot->other_end = b;
+  *out = slot;
+  return DG3D_OK;
+}
+
 int reduce_partials(dg3d_ctx* c, int per_block, int blocks) {
   c->h_part.resize((size_t)per_block * blocks);
   hipError_t e = hipMemcpyAsync(c->h_part.data(), c->d_part, sizeof(double) * c->h_part.size(),
@@ -290,6 +382,7 @@ int reduce_partials(dg3d_ctx* c, int per_block, int blocks) {
 // registers, which the next step's first stage overwrites without reading, so tuning leaves the
 // solution unchanged. The fastest pair is kept.
 int autotune(dg3d_ctx* c, int scheme) {
+  drop_graphs(c);
   int rc = ensure_work(c, scheme);
   if (rc) return rc;
   if (scheme == DG3D_LSRK54) {  // give the du register defined contents before it is read
@@ -368,8 +461,16 @@ int dg3d_set_variant(dg3d_ctx* c, int scheme, int variant, int xcd_remap) {
   if (!c || (scheme != DG3D_LSRK54 && scheme != DG3D_RK33)) return c ? c->fail(DG3D_E_INVALID, "set_variant: bad scheme") : DG3D_E_INVALID;
   if (variant < 0 || variant >= dg3d::kNumVariants) return c->fail(DG3D_E_INVALID, "set_variant: variant out of range");
   if (xcd_remap < 0 || xcd_remap >= dg3d::kNumOrders) return c->fail(DG3D_E_INVALID, "set_variant: order out of range");
+  drop_graphs(c);
   c->tune_variant[scheme] = variant;
   c->tune_xcd[scheme] = xcd_remap;
+  return DG3D_OK;
+}
+
+int dg3d_set_graphs(dg3d_ctx* c, int enable) {
+  if (!c) return DG3D_E_INVALID;
+  drop_graphs(c);
+  c->use_graphs = enable != 0;
   return DG3D_OK;
 }
 
@@ -460,12 +561,15 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
     }
     c->h_diff.assign((size_t)c->E, 0.0);
     if (dmalloc(&c->d_mat, (size_t)c->E * dg3d::kMatStride) != hipSuccess || dmalloc(&c->d_diff, c->h_diff.size()) != hipSuccess ||
-        dmalloc(&c->d_wq, c->n) != hipSuccess || dmalloc(&c->d_part, 1024 * 16) != hipSuccess) {
+        dmalloc(&c->d_wq, c->n) != hipSuccess || dmalloc(&c->d_part, 1024 * 16) != hipSuccess ||
+        dmalloc(&c->d_clock, 1) != hipSuccess || dmalloc(&c->d_probe_step, 1) != hipSuccess) {
       g_create_error = "material allocation failed";
       free_ctx(c);
       return DG3D_E_NOMEM;
     }
     hipMemcpy(c->d_wq, c->weights.data(), sizeof(double) * c->n, hipMemcpyHostToDevice);
+    hipMemset(c->d_clock, 0, sizeof(double));
+    hipMemset(c->d_probe_step, 0, sizeof(int));
     if (upload_material(c) != DG3D_OK) { g_create_error = c->err; free_ctx(c); return DG3D_E_HIP; }
     c->regs.resize(d->num_registers);
     for (auto& r : c->regs) {
@@ -523,6 +627,7 @@ int dg3d_set_material(dg3d_ctx* c, const double* c0, const double* rho0, const d
     if (!(c->h_mat[i * 4 + 0] > 0.0) || !(c->h_mat[i * 4 + 1] > 0.0))
       return c->fail(DG3D_E_INVALID, "c0 and rho0 must be positive");
   }
+  drop_graphs(c);  // whether the stages run the LDG term depends on D
   return upload_material(c);
 }
 
@@ -663,6 +768,7 @@ int dg3d_halo(dg3d_ctx* c, int reg) {
 int dg3d_set_source(dg3d_ctx* c, const dg3d_source* s) {
   if (!c) return DG3D_E_INVALID;
   hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   hipFree(c->d_lead);
   hipFree(c->d_src_trace);
   c->d_lead = nullptr;
@@ -733,23 +839,29 @@ int dg3d_step(dg3d_ctx* c, int scheme, double dt, int nsteps, double* t) {
   double* other = c->work_q;
   double tt = *t;
   hipEventRecord(c->ev0, c->stream);
-  bool first = true;
+  hipError_t e = hipSuccess;
+  if (c->src.face >= 0) e = dg3d::launch_clock_set(c->d_clock, tt, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "step");
+  bool halos = false;
+  for (bool v : c->regs[0].halo_valid) halos = halos || v;
   for (int s = 0; s < nsteps; ++s) {
-    for (int st = 0; st < ns; ++st) {
-      rc = run_stage(c, scheme, st, dt, tt, &cur, &other, first);
+    if (!c->use_graphs || (s == 0 && halos)) {
+      // the first stage reads the halo traces set on register 0 (initial-condition halos)
+      rc = launch_step(c, scheme, dt, &cur, &other, s == 0);
       if (rc) return rc;
-      if (first) {
+      if (s == 0)
         for (bool& v : c->regs[0].halo_valid) v = false;
-        first = false;
-      }
+    } else {
+      StepGraph* g = nullptr;
+      rc = step_graph(c, scheme, dt, cur, other, &g);
+      if (rc) return rc;
+      e = hipGraphLaunch(g->exec, c->stream);
+      if (e != hipSuccess) return c->hip_fail(e, "step graph launch");
+      cur = g->cur_end;
+      other = g->other_end;
     }
     tt += dt;
-    if (c->probe_count > 0 && c->probe_steps < c->probe_cap) {
-      hipError_t e = dg3d::launch_probe(cur, c->mesh(), c->n, c->d_probe_nodes, c->probe_count,
-                                        c->d_probe_buf + (size_t)c->probe_steps * c->probe_count, c->stream);
-      if (e != hipSuccess) return c->hip_fail(e, "probe");
-      ++c->probe_steps;
-    }
+    if (c->probe_count > 0 && c->probe_steps < c->probe_cap) ++c->probe_steps;
   }
   hipEventRecord(c->ev1, c->stream);
   if (scheme == DG3D_LSRK54 && cur != c->regs[0].d) {
@@ -759,7 +871,7 @@ int dg3d_step(dg3d_ctx* c, int scheme, double dt, int nsteps, double* t) {
     c->work_q = c->regs[0].d;
     c->regs[0].d = cur;
   }
-  hipError_t e = hipEventSynchronize(c->ev1);
+  e = hipEventSynchronize(c->ev1);
   if (e != hipSuccess) return c->hip_fail(e, "step");
   float ms = 0.f;
   hipEventElapsedTime(&ms, c->ev0, c->ev1);
@@ -850,6 +962,10 @@ int dg3d_reduce(dg3d_ctx* c, int kind, int reg, int reg2, double* out) {
 
 int dg3d_set_probes(dg3d_ctx* c, int count, const int64_t* nodes, int capacity_steps) {
   if (!c || count < 0 || capacity_steps < 0) return DG3D_E_INVALID;
+  hipStreamSynchronize(c->stream);
+  drop_graphs(c);
+  hipError_t e0 = hipMemsetAsync(c->d_probe_step, 0, sizeof(int), c->stream);
+  if (e0 != hipSuccess) return c->hip_fail(e0, "set_probes");
   hipFree(c->d_probe_nodes);
   hipFree(c->d_probe_buf);
   c->d_probe_nodes = nullptr;

@@ -94,6 +94,7 @@ _SIGS = {
     "dg3d_cfl_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _dp]),
     "dg3d_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip]),
     "dg3d_set_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "dg3d_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "dg3d_reduce": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "dg3d_set_probes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
                                         ctypes.c_int]),
@@ -313,6 +314,10 @@ class Context:
 
     def set_variant(self, scheme, variant, xcd_remap):
         self._check(self._L.dg3d_set_variant(self._h, scheme, int(variant), int(xcd_remap)), "set_variant")
+
+    def set_graphs(self, enable):
+        """dg3d_step replays captured HIP graphs (True, the default) or launches directly."""
+        self._check(self._L.dg3d_set_graphs(self._h, 1 if enable else 0), "set_graphs")
 
     def cfl_dt(self, cfl):
         v = ctypes.c_double()

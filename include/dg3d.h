@@ -188,6 +188,12 @@ int dg3d_set_probes(dg3d_ctx* ctx, int count, const int64_t* node_index, int cap
 int dg3d_get_probes(dg3d_ctx* ctx, double* out, int* steps_recorded);
 
 int dg3d_sync(dg3d_ctx* ctx);
+/* Launch mode of dg3d_step (no counterpart in the reference; default on): 1 replays each step
+ * as a captured HIP graph, one per (scheme, dt, state buffer), captured on first use and
+ * dropped by dg3d_set_variant, dg3d_autotune, dg3d_set_material, dg3d_set_source and
+ * dg3d_set_probes; 0 launches every kernel directly. The results do not depend on the mode. */
+int dg3d_set_graphs(dg3d_ctx* ctx, int enable);
+
 /* Average device time of the last dg3d_step's stage kernels in ms (HIP events on the
  * context stream), and the number of stage launches it averaged. */
 int dg3d_last_step_timing(dg3d_ctx* ctx, double* avg_stage_ms, int* launches);

@@ -1,0 +1,75 @@
+"""dg3d_step replays each step as a captured HIP graph. The replays must equal direct launches bit
+for bit, call after call, with the program's own work and synchronisations between the calls:
+boundary source with per-node delays (device clock), probes (device row counter), initial-condition
+halos (first step launched directly), LDG diffusion (Q kernel in the graph), both schemes, both
+precisions, and graphs dropped by a variant change and by a new dt."""
+import numpy as np
+import pytest
+
+from helpers import dg3d, heterogeneous, problem, small_c1, small_c2, small_c5
+
+pytestmark = pytest.mark.gpu
+
+
+def _probe_nodes(p, count=5):
+    rng = np.random.default_rng(11)
+    return rng.integers(0, p.dof, size=count).astype(np.int64)
+
+
+def _calls(p, scheme, precision, graphs, with_q=False):
+    """Several dg3d_step calls of uneven length; the state, time and probes after each."""
+    ctx = problem.make_context(p, precision=precision, with_q=with_q)
+    ctx.set_graphs(graphs)
+    ctx.set_probes(_probe_nodes(p), 64)
+    dt = problem.cfl_dt_host(p)
+    t = 0.0
+    out = []
+    for i, n in enumerate([1, 4, 3, 2, 5]):
+        if i == 3:
+            ctx.set_variant(scheme, 1, 3)   # drops the captured graphs
+        step_dt = dt if i < 4 else 0.5 * dt
+        t = ctx.step(scheme, step_dt, n, t)
+        energy = ctx.reduce(dg3d.REDUCE_ENERGY, 0)   # eager work between the replays
+        out.append((ctx.get_state(0), t, energy[0]))
+    probes = ctx.get_probes()
+    ctx.close()
+    return out, probes
+
+
+def _check(p, scheme, precision, with_q=False):
+    direct, pd = _calls(p, scheme, precision, False, with_q)
+    graph, pg = _calls(p, scheme, precision, True, with_q)
+    for i, ((sd, td, ed), (sg, tg, eg)) in enumerate(zip(direct, graph)):
+        assert np.linalg.norm(sd) > 0
+        assert td == tg, (i, td, tg)
+        assert np.array_equal(sd, sg), (i, np.abs(sd - sg).max())
+        assert ed == eg, (i, ed, eg)
+    assert len(pd) == len(pg) == 15
+    assert np.array_equal(np.asarray(pd), np.asarray(pg))
+
+
+@pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
+@pytest.mark.parametrize("precision", [dg3d.FP64, dg3d.MIXED])
+def test_graphs_burst_source(scheme, precision):
+    print(f"graphs_burst_source scheme={scheme} precision={precision}", flush=True)
+    _check(heterogeneous(small_c2(n_e=4, order=3)), scheme, precision)
+
+
+def test_graphs_focused_source_with_delays():
+    print("graphs_focused_source", flush=True)
+    _check(small_c5(n_e=4), dg3d.LSRK54, dg3d.FP64)
+
+
+@pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
+def test_graphs_ic_halos(scheme):
+    """The Gaussian IC's halo traces are read by the first stage of the first call only."""
+    print(f"graphs_ic_halos scheme={scheme}", flush=True)
+    _check(small_c1(), scheme, dg3d.FP64)
+
+
+@pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
+def test_graphs_ldg_diffusion(scheme):
+    print(f"graphs_ldg scheme={scheme}", flush=True)
+    p = heterogeneous(small_c2(n_e=3, order=2))
+    p.diffusion = np.full(p.E, 1.0e-3)
+    _check(p, scheme, dg3d.FP64, with_q=True)
