@@ -24,18 +24,23 @@ using dg3d::kLsC;
 
 thread_local std::string g_create_error;
 
-// One captured step of dg3d_step: the launches of the step (stages, probe, clock), keyed by the
-// registers it starts from and dt; everything else it reads is fixed until a setter drops it.
+// A captured run of `steps` steps of dg3d_step (each: stages, probe, clock), keyed by the
+// registers it starts from, dt and the step count; everything else it reads is fixed until a
+// setter drops it.
 struct StepGraph {
   hipGraphExec_t exec = nullptr;
   int scheme = -1;
+  int steps = 0;
   double dt = 0.0;
   double* cur = nullptr;    // state register the step reads
   double* other = nullptr;  // LSRK54: the second state buffer
   double* cur_end = nullptr;
   double* other_end = nullptr;
 };
-constexpr int kMaxStepGraphs = 4;
+constexpr int kMaxStepGraphs = 16;
+// dg3d_step replays runs of kGraphSteps steps, and one shorter run (a power of two each) for the
+// remainder.
+constexpr int kGraphSteps = 16;
 
 struct Register {
   double* d = nullptr;          // nfields * fs doubles
@@ -326,10 +331,11 @@ int launch_step(dg3d_ctx* c, int scheme, double dt, double** cur, double** other
   return DG3D_OK;
 }
 
-// The captured form of launch_step for (scheme, dt, cur, other), captured on first use.
-int step_graph(dg3d_ctx* c, int scheme, double dt, double* cur, double* other, StepGraph** out) {
+// The captured form of `steps` launch_step calls for (scheme, dt, cur, other), captured on first
+// use.
+int step_graph(dg3d_ctx* c, int scheme, double dt, int steps, double* cur, double* other, StepGraph** out) {
   for (StepGraph& g : c->graphs)
-    if (g.exec && g.scheme == scheme && g.dt == dt && g.cur == cur && g.other == other) {
+    if (g.exec && g.scheme == scheme && g.steps == steps && g.dt == dt && g.cur == cur && g.other == other) {
       *out = &g;
       return DG3D_OK;
     }
@@ -345,7 +351,8 @@ int step_graph(dg3d_ctx* c, int scheme, double dt, double* cur, double* other, S
   if (e != hipSuccess) return c->hip_fail(e, "step graph capture");
   double* a = cur;
   double* b = other;
-  const int rc = launch_step(c, scheme, dt, &a, &b, false);
+  int rc = DG3D_OK;
+  for (int i = 0; i < steps && !rc; ++i) rc = launch_step(c, scheme, dt, &a, &b, false);
   e = hipStreamEndCapture(c->stream, &graph);
   if (rc || e != hipSuccess) {
     if (graph) hipGraphDestroy(graph);
@@ -358,6 +365,7 @@ int step_graph(dg3d_ctx* c, int scheme, double dt, double* cur, double* other, S
     return c->hip_fail(e, "step graph instantiate");
   }
   slot->scheme = scheme;
+  slot->steps = steps;
   slot->dt = dt;
   slot->cur = cur;
   slot->other = other;
@@ -844,7 +852,8 @@ int dg3d_step(dg3d_ctx* c, int scheme, double dt, int nsteps, double* t) {
   if (e != hipSuccess) return c->hip_fail(e, "step");
   bool halos = false;
   for (bool v : c->regs[0].halo_valid) halos = halos || v;
-  for (int s = 0; s < nsteps; ++s) {
+  for (int s = 0; s < nsteps;) {
+    int run = 1;
     if (!c->use_graphs || (s == 0 && halos)) {
       // the first stage reads the halo traces set on register 0 (initial-condition halos)
       rc = launch_step(c, scheme, dt, &cur, &other, s == 0);
@@ -852,16 +861,21 @@ int dg3d_step(dg3d_ctx* c, int scheme, double dt, int nsteps, double* t) {
       if (s == 0)
         for (bool& v : c->regs[0].halo_valid) v = false;
     } else {
+      run = kGraphSteps;
+      while (run > nsteps - s) run /= 2;
       StepGraph* g = nullptr;
-      rc = step_graph(c, scheme, dt, cur, other, &g);
+      rc = step_graph(c, scheme, dt, run, cur, other, &g);
       if (rc) return rc;
       e = hipGraphLaunch(g->exec, c->stream);
       if (e != hipSuccess) return c->hip_fail(e, "step graph launch");
       cur = g->cur_end;
       other = g->other_end;
     }
-    tt += dt;
-    if (c->probe_count > 0 && c->probe_steps < c->probe_cap) ++c->probe_steps;
+    for (int i = 0; i < run; ++i) {
+      tt += dt;
+      if (c->probe_count > 0 && c->probe_steps < c->probe_cap) ++c->probe_steps;
+    }
+    s += run;
   }
   hipEventRecord(c->ev1, c->stream);
   if (scheme == DG3D_LSRK54 && cur != c->regs[0].d) {

@@ -171,17 +171,21 @@ int main(int argc, char* argv[]) {
   }
   const unsigned NUM_TIMESTEPS = p.NUM_OUTSTEPS * p.NUM_TIME_SUBSTEPS;
   std::cout << "Creating mesh\n";
-  const bool fused = p.FUSED && p.eval_advect;
+  // SCHEME = EULER runs Grid::forwardEuler, which exists only on the Block-operator path
+  const bool fused = p.FUSED && p.eval_advect && p.SCHEME != 2;
   if (!fused && p.SCHEME == 1) {
     std::cerr << "SCHEME = LSRK54 runs on the fused path only (advection on)\n";
     return ERRCODE_INVALID_PARAMETER;
   }
   int periodic[3] = {p.IS_PERIODIC[0], p.IS_PERIODIC[1], p.IS_PERIODIC[2]};
   try {
-    Solver solver(mesh, periodic, p.PRECISION, fused ? 1 : 4, p.eval_diff);
+    Solver solver(mesh, periodic, p.PRECISION, fused ? 1 : 4, p.eval_diff || p.SCHEME == 2);
     std::vector<double> c0, rho0, ba, sigma, diff, st;
     material_from_params(p, mesh, &c0, &rho0, &ba, &sigma, &diff);
-    solver.check(dg3d_set_material(solver.ctx(), c0.data(), rho0.data(), ba.data(), sigma.data(), diff.data()),
+    // D enters the stages only where the reference evaluates diffusion: Grid::RK33 under
+    // EVALUATE_DIFFUSION (Grid.cpp:655), Grid::forwardEuler always; the grid dump keeps DIFF_CONST
+    const std::vector<double> d_used = (p.eval_diff || p.SCHEME == 2) ? diff : std::vector<double>(diff.size(), 0.0);
+    solver.check(dg3d_set_material(solver.ctx(), c0.data(), rho0.data(), ba.data(), sigma.data(), d_used.data()),
                  "set_material");
     std::vector<double> traces[6];
     gaussian_ic(p, mesh, &st, traces);
@@ -268,7 +272,8 @@ int main(int argc, char* argv[]) {
           swapGrid = oldGrid;
           oldGrid = newGrid;
           newGrid = swapGrid;
-          newGrid->RK33(DT, oldGrid, &k1, &k2, stepLog);
+          if (p.SCHEME == 2) newGrid->forwardEuler(DT, oldGrid, stepLog);
+          else newGrid->RK33(DT, oldGrid, &k1, &k2, stepLog);
           t += DT;
         }
       }

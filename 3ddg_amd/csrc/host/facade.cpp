@@ -5,7 +5,8 @@
 
 namespace dg3d_host {
 
-Solver::Solver(const Mesh& mesh, const int* periodic, int precision, int registers, bool with_q) : mesh_(mesh) {
+Solver::Solver(const Mesh& mesh, const int* periodic, int precision, int registers, bool with_q)
+    : mesh_(mesh), with_q_(with_q) {
   dg3d_desc d{};
   d.order = mesh.order;
   for (int a = 0; a < 3; ++a) {
@@ -92,6 +93,25 @@ void Grid::RK33(double dt, Grid* oldGrid, Grid* k1, Grid* k2, std::ofstream& log
       mBlock.selfWeightedAdd(&oldGrid->mBlock, 1.0, 1.0 / 3.0);
     }
   }
+}
+
+void Grid::forwardEuler(double dt, Grid* oldGrid, std::ofstream& logFile) {
+  (void)logFile;
+  if (!solver_->with_q())
+    throw std::runtime_error("Grid::forwardEuler runs the x-axis LDG pair: the Solver needs Q fields");
+  mTime += dt;
+  oldGrid->mTime += dt;
+  setZero();
+  oldGrid->communicate();
+  oldGrid->mBlock.OperateQFromDRho(0, 0, &oldGrid->mBlock);
+  oldGrid->communicate();
+  mBlock.OperateDRhoFromQ(0, 0, &oldGrid->mBlock);
+  for (unsigned short a = 0; a < 3; ++a) {
+    mBlock.OperateUFromV(a, 0, &oldGrid->mBlock);
+    mBlock.OperateUFromV(a, 1, &oldGrid->mBlock);
+  }
+  mBlock.OperateSponge(0, 0, &oldGrid->mBlock);
+  mBlock.selfWeightedAdd(&oldGrid->mBlock, dt, 1.0);
 }
 
 bool load_basis(int order, const std::vector<std::string>& dirs, Mesh* m, std::string* err) {

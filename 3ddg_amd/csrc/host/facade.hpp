@@ -48,6 +48,9 @@ class Grid {
   Grid(Solver* s, int reg, bool evalAdvect, bool evalDiff, bool evalSponge);
   // Grid::RK33 (Grid.cpp:642-902) through the Block operators.
   void RK33(double dt, Grid* oldGrid, Grid* k1, Grid* k2, std::ofstream& logFile);
+  // Grid::forwardEuler (Grid.cpp:567-640): ignores the evaluate flags, as the reference does;
+  // the LDG pair runs for the x axis only. Needs a Solver with Q fields.
+  void forwardEuler(double dt, Grid* oldGrid, std::ofstream& logFile);
   void communicate();
   void setZero();
   void copyFrom(const Grid& g);
@@ -66,12 +69,14 @@ class Solver {
   Solver(const Solver&) = delete;
   Solver& operator=(const Solver&) = delete;
   dg3d_ctx* ctx() const { return ctx_; }
+  bool with_q() const { return with_q_; }
   void check(int rc, const char* what) const;
   const Mesh& mesh() const { return mesh_; }
 
  private:
   dg3d_ctx* ctx_ = nullptr;
   Mesh mesh_;
+  bool with_q_ = false;
 };
 
 // Problem setup from reference parameters (global mesh of NUM_BLOCKS x NUM_ELEMENTS elements).

@@ -55,7 +55,7 @@ bool parse_parameters(const std::string& path, Params* P, std::ostream& log, std
     else if (name == "SCHEME") {
       std::string v;
       iss >> v;
-      p.SCHEME = (v == "LSRK54" || v == "1") ? 1 : 0;
+      p.SCHEME = (v == "LSRK54" || v == "1") ? 1 : ((v == "EULER" || v == "2") ? 2 : 0);
     } else if (name == "CFL") iss >> p.CFL;
     else if (name == "SPONGE_RAMP") iss >> p.SPONGE_RAMP;
     else if (name == "PRECISION") {

@@ -37,7 +37,7 @@ struct Params {
 
   // Keys this implementation adds (the reference warns on unknown keys and ignores them, so
   // files using them stay readable by both).
-  int SCHEME = 0;          // 0 = RK33 (reference), 1 = LSRK54
+  int SCHEME = 0;          // 0 = RK33 (reference), 1 = LSRK54, 2 = EULER (Grid::forwardEuler)
   float CFL = 0.0F;        // > 0: dt from the d-dimensional CFL condition instead of DT
   int SPONGE_RAMP = 0;     // 0 = linear (reference), 1 = quadratic
   int PRECISION = 0;       // 0 = FP64, 1 = mixed (FP64 state, FP32 right-hand side)

@@ -92,6 +92,34 @@ def timed_run(pk, p, precision, steps, warmup, probe=None, variant=-1, xcd=1, co
     return ctx, dict(elapsed=el, stage_ms=stage_ms, launches=launches, dt=dt, t_end=t, variant=variant, xcd=xcd)
 
 
+def small_block_run(pk, steps=2000, warmup=20):
+    """The 8^3 N=3 LSRK54 block (32 768 nodes, Gaussian pulse, IC halos): kernels of a few
+    microseconds, so the rate is set by launches. Times dg3d_step with its step graphs and with
+    direct launches (wall clock over the timed steps)."""
+    import torch
+    p = pk.problem.config_c1(steps=steps)
+    out = {"workload": "c1", "elements": list(p.num_elements), "order": p.order, "dof": p.dof, "steps": steps}
+    for graphs in (True, False):
+        ctx = pk.problem.make_context(p)
+        ctx.set_graphs(graphs)
+        dt = ctx.cfl_dt(p.cfl)
+        v, x = ctx.autotune(pk.dg3d.LSRK54)
+        t = ctx.step(pk.dg3d.LSRK54, dt, warmup, 0.0)
+        ctx.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t = ctx.step(pk.dg3d.LSRK54, dt, steps, t)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        key = "graph" if graphs else "direct"
+        out[f"{key}_dof_stage_per_s"] = p.dof * 5 * steps / el
+        out[f"{key}_us_per_step"] = el / steps * 1e6
+        out[f"{key}_variant"] = [v, x]
+        ctx.close()
+    out["graph_speedup"] = out["graph_dof_stage_per_s"] / out["direct_dof_stage_per_s"]
+    return out
+
+
 def probe_node(p, depth_frac=0.05):
     """Global node index on the axis (x, y at the centre) near the source face."""
     n = p.n
@@ -293,6 +321,8 @@ def main():
                                              args.steps, args.warmup)
         out["mixed_c3"] = compare_precisions(pk, pk.problem.config_c3(num_elements=args.elements, steps=args.steps),
                                              args.steps, args.warmup)
+    if not args.no_extra:
+        out["small_c1"] = small_block_run(pk)
     if not args.no_batch and not args.no_extra:
         out["batch_c4"] = batch_run(pk, args.batch_steps)
     if not args.no_extra and args.c5_steps > 0:

@@ -739,6 +739,25 @@ void oracle_rk33(oracle* o, double dt, int nw, int old, int k1, int k2, int adv,
   o->time = t + dt;
 }
 
+void oracle_forward_euler(oracle* o, double dt, int nw, int old) {
+  // Grid::forwardEuler single-block branch (Grid.cpp:567-640): no evaluate flags; the LDG pair
+  // runs for the x axis only (Grid.cpp:581,597), then advection for the three axes, the sponge
+  // and new = L(old) dt + old.
+  const double t = o->time;
+  oracle_zero(o, nw);
+  oracle_communicate(o, old);
+  oracle_operate(o, 2, 0, 0, old, old);
+  oracle_communicate(o, old);
+  oracle_operate(o, 3, 0, 0, old, nw);
+  for (int a = 0; a < 3; ++a) {
+    oracle_operate(o, 0, a, 0, old, nw);
+    oracle_operate(o, 0, a, 1, old, nw);
+  }
+  oracle_operate(o, 1, 0, 0, old, nw);
+  oracle_axpby(o, nw, o->literal ? (double)(float)dt : dt, old, 1.0);
+  o->time = t + dt;
+}
+
 void oracle_lsrk54(oracle* o, double dt, int q, int du, int rhs, int adv, int diff, int sponge) {
   // 2N-storage: du = A_s du + dt L(q, t + C_s dt); q = q + B_s du (interior only).
   const double t = o->time;

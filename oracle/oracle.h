@@ -75,6 +75,9 @@ void oracle_set_time(oracle* o, double t);
 /* Grid::RK33 (Grid.cpp:642-835): new_reg = RK33(old_reg) using k1, k2 registers. */
 void oracle_rk33(oracle* o, double dt, int new_reg, int old_reg, int k1, int k2, int eval_advect,
                  int eval_diff, int eval_sponge);
+/* Grid::forwardEuler (Grid.cpp:567-640): new_reg = old_reg + dt L(old_reg), with the LDG pair on
+ * the x axis only, advection and sponge always on. Writes Q of old_reg. */
+void oracle_forward_euler(oracle* o, double dt, int new_reg, int old_reg);
 /* One LSRK(5,4) step on q_reg with du_reg and a scratch register rhs_reg. */
 void oracle_lsrk54(oracle* o, double dt, int q_reg, int du_reg, int rhs_reg, int eval_advect,
                    int eval_diff, int eval_sponge);

@@ -58,6 +58,7 @@ def lib():
             "oracle_set_time": (None, [ctypes.c_void_p, ctypes.c_double]),
             "oracle_rk33": (None, [ctypes.c_void_p, ctypes.c_double] + [ctypes.c_int] * 7),
             "oracle_lsrk54": (None, [ctypes.c_void_p, ctypes.c_double] + [ctypes.c_int] * 6),
+            "oracle_forward_euler": (None, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
             "oracle_maxabs": (None, [ctypes.c_void_p, ctypes.c_int, _dp]),
             "oracle_energy": (None, [ctypes.c_void_p, ctypes.c_int, _dp]),
             "oracle_sponge_sigma": (None, [_ip, _ip, _ip, _dp, _ip, ctypes.c_double, ctypes.c_int, _dp]),
@@ -167,6 +168,9 @@ class Oracle:
 
     def rk33(self, dt, new, old, k1, k2, adv=1, diff=0, sponge=1):
         self.L.oracle_rk33(self.h, float(dt), new, old, k1, k2, adv, diff, sponge)
+
+    def forward_euler(self, dt, new, old):
+        self.L.oracle_forward_euler(self.h, float(dt), new, old)
 
     def lsrk54(self, dt, q, du, rhs, adv=1, diff=0, sponge=1):
         self.L.oracle_lsrk54(self.h, float(dt), q, du, rhs, adv, diff, sponge)

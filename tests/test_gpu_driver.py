@@ -28,9 +28,10 @@ def _params(tmp_path, extra):
     return pf
 
 
-def _oracle_reference_run(steps, diffusion=True):
-    """The same run on the oracle: 16x16x16 elements (2x2x1 blocks of 8x8x16), N=3, RK33,
-    DT=1e-6, B/A=5, D=5, linear sponge sigma_max 15000 width 1, Gaussian IC at the origin."""
+def _oracle_reference_run(steps, diffusion=True, euler=False):
+    """The same run on the oracle: 16x16x16 elements (2x2x1 blocks of 8x8x16), N=3, RK33 (or
+    Grid::forwardEuler), DT=1e-6, B/A=5, D=5, linear sponge sigma_max 15000 width 1, Gaussian IC
+    at the origin."""
     nE = (16, 16, 16)
     p = problem._base("p1", 3, nE, (-1.0, -1.0, -1.0), problem._uniform_widths(nE, (0.125,) * 3), 1500.0, 1000.0, 5.0)
     p.diffusion = np.full(p.E, 5.0)
@@ -42,7 +43,10 @@ def _oracle_reference_run(steps, diffusion=True):
     o.fill_ic(0)
     dt = float(np.float32(0.000001))
     for _ in range(steps):
-        o.rk33(dt, 3, 0, 1, 2, adv=1, diff=1 if diffusion else 0, sponge=1)
+        if euler:
+            o.forward_euler(dt, 3, 0)
+        else:
+            o.rk33(dt, 3, 0, 1, 2, adv=1, diff=1 if diffusion else 0, sponge=1)
         o.copy(0, 3)
     return p, o.get_state(0)
 
@@ -65,10 +69,16 @@ def _gather(p, out_dir, step):
 
 
 @pytest.mark.skipif(not h5read.available(), reason="HDF5 library not available")
-@pytest.mark.parametrize("fused", [1, 0])
-def test_driver_parameters1_matches_oracle(tmp_path, fused):
-    print(f"driver_parameters1_matches_oracle fused={fused}", flush=True)
-    pf = _params(tmp_path, f"FUSED = {fused}\n")
+@pytest.mark.parametrize("case", ["fused", "operators", "fused_no_diffusion", "euler"])
+def test_driver_parameters1_matches_oracle(tmp_path, case):
+    """FUSED = 1 (fused stages) and FUSED = 0 (Grid::RK33 through the Block operators) with
+    diffusion; EVALUATE_DIFFUSION = 0 on the fused path with DIFF_CONST still 5 (the stages must
+    leave diffusion out); SCHEME = EULER (Grid::forwardEuler, which always runs the x-axis LDG
+    pair)."""
+    print(f"driver_parameters1_matches_oracle case={case}", flush=True)
+    extra = {"fused": "FUSED = 1\n", "operators": "FUSED = 0\n", "euler": "SCHEME = EULER\n",
+             "fused_no_diffusion": "FUSED = 1\nEVALUATE_DIFFUSION = 0\n"}[case]
+    pf = _params(tmp_path, extra)
     out = tmp_path / "out"
     r = subprocess.run([DRIVER, str(pf), str(out)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
@@ -79,7 +89,7 @@ def test_driver_parameters1_matches_oracle(tmp_path, fused):
     c = h5read.read(out / "raw" / "coords_0_0.h5", "coords")
     assert c.shape == (3, 32, 32, 64)
     assert abs(c[0, 0, 0, 0] - (-1.0)) < 1e-12 and abs(c[2, 0, 0, -1] - 1.0) < 1e-12
-    p, ref = _oracle_reference_run(6)
+    p, ref = _oracle_reference_run(6, diffusion=case != "fused_no_diffusion", euler=case == "euler")
     got = _gather(p, out, 6)
     a = got.reshape(4, -1)
     b = ref.reshape(4, -1)

@@ -330,3 +330,92 @@ def test_oracle_ldg_heat_kernel_decay():
     assert decay < 0.7          # a decay that a sign or scale error of D would miss
     assert e6 < 5e-6, e6
     assert e4 / e6 > 50, (e4, e6)
+
+
+# ---- Grid::forwardEuler (Grid.cpp:567-640) --------------------------------------------
+
+def _periodic_box(N=3, ne=4, D=0.0):
+    nE = (ne, ne, ne)
+    p = problem._base("box", N, nE, (0, 0, 0), problem._uniform_widths(nE, (1.0 / ne,) * 3), 1500.0, 1000.0, 0.0)
+    p.periodic = (1, 1, 1)
+    p.diffusion = np.full(p.E, D)
+    return p
+
+
+def test_oracle_forward_euler_order_one_on_sponge_decay():
+    """y' = -sigma y for a uniform rho' in a periodic box (advection and LDG vanish): one step
+    gives (1 - sigma dt) y exactly, and the error at a fixed time halves with dt (order 1)."""
+    p = _periodic_box(N=2, ne=2)
+    sig = 1000.0
+    p.sigma = np.full(p.E, sig)
+    T = 1e-3
+    errs = []
+    for steps in (1, 16, 32, 64):
+        o = ob.Oracle(p, num_registers=2)
+        st = np.zeros(7 * p.dof)
+        st[: p.dof] = 1.0
+        o.set_state(0, st)
+        dt = T / steps
+        cur, nxt = 0, 1
+        for _ in range(steps):
+            o.forward_euler(dt, nxt, cur)
+            cur, nxt = nxt, cur
+        y = o.get_state(cur)[: p.dof]
+        if steps == 1:
+            assert np.abs(y - (1.0 - sig * dt)).max() < 1e-14
+        else:
+            errs.append(abs(y.mean() - np.exp(-sig * T)))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(np.abs(rates - 1.0) < 0.1), rates
+
+
+def test_oracle_forward_euler_is_rk33_first_stage():
+    """With D = 0, forwardEuler's step is Grid::RK33's first stage k1 = L(u0) dt + u0
+    (Grid.cpp:705): a Gaussian pulse with the initial-condition halos, sponge, B/A = 5."""
+    p = small_c1()
+    p.b_over_a = np.full(p.E, 5.0)
+    dt = problem.cfl_dt_host(p)
+    a = ob.Oracle(p, num_registers=2)
+    ob.load_problem_into(a, p, 0)
+    a.set_time(0.0)
+    a.forward_euler(dt, 1, 0)
+    b = ob.Oracle(p, num_registers=4)
+    ob.load_problem_into(b, p, 0)
+    b.set_time(0.0)
+    b.rk33(dt, 3, 0, 1, 2, adv=1, diff=0, sponge=1)
+    got, k1 = a.get_state(1), b.get_state(1)
+    assert np.linalg.norm(k1 - b.get_state(0)) > 0
+    assert np.abs(got - k1).max() <= 1e-15 * np.abs(k1).max()
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_oracle_forward_euler_ldg_on_x_only(axis):
+    """forwardEuler runs the LDG pair for the x axis only (Grid.cpp:581,597). For a mode along x
+    its step equals RK33's first stage with the full three-axis LDG; for a mode along y or z the
+    diffusion leaves it unchanged, while the three-axis stage changes it visibly."""
+    p = _periodic_box(N=4, ne=4, D=1.0)
+    n = p.n
+    coords = problem.axis_coords(p, axis)
+    shape = tuple(p.num_elements) + (n, n, n)
+    idx = [None] * 6
+    idx[axis] = slice(None)
+    idx[3 + axis] = slice(None)
+    xa = np.broadcast_to(coords[tuple(idx[a] if a in (axis, 3 + axis) else None for a in range(6))], shape)
+    st = np.zeros((7,) + shape)
+    st[0] = 1e-4 * np.sin(2 * np.pi * xa)
+    dt = problem.cfl_dt_host(p)
+    a = ob.Oracle(p, num_registers=2)
+    a.set_state(0, st.reshape(-1))
+    a.forward_euler(dt, 1, 0)
+    b = ob.Oracle(p, num_registers=4)
+    b.set_state(0, st.reshape(-1))
+    b.rk33(dt, 3, 0, 1, 2, adv=1, diff=1, sponge=1)
+    e_rho = a.get_state(1)[: p.dof]
+    k1_rho = b.get_state(1)[: p.dof]
+    rho0 = st[0].reshape(-1)
+    change = np.abs(k1_rho - rho0).max()
+    assert change > 1e-4 * 1e-4          # the diffusion step is far above rounding
+    if axis == 0:
+        assert np.abs(e_rho - k1_rho).max() < 1e-9 * change
+    else:
+        assert np.abs(e_rho - rho0).max() < 1e-9 * change
