@@ -67,9 +67,11 @@ struct dg3d_batch {
     int64_t nodes = 0;
     hipStream_t stream = nullptr;      // branch of the stage's fork/join
     hipEvent_t done = nullptr;
+    int branch = -1;                   // stream index this order runs on (-1: the batch stream)
   };
   std::vector<Group> groups;
   bool tuned = false;
+  int schedule = 0;  // how a stage runs the orders (kScheduleBranches, kScheduleSerial, kScheduleTwo)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // one step starting from arena 0 / arena 1
@@ -134,18 +136,44 @@ dg3d::BatchLaunch stage_launch(const dg3d_batch* b, const dg3d_batch::Group& g, 
   return B;
 }
 
-// One stage of every solve: the per-order persistent launches run as parallel branches
-// (fork/join on events), so the orders share the device.
+// Stage schedules: every order on a branch of its own (the orders share the device); all orders
+// one after another on the batch stream, largest first (each has the device to itself); or two
+// branches with about the same number of nodes.
+constexpr int kScheduleBranches = 0, kScheduleSerial = 1, kScheduleTwo = 2, kNumSchedules = 3;
+
+void set_schedule(dg3d_batch* b, int schedule) {
+  b->schedule = schedule;
+  int64_t load[2] = {0, 0};
+  for (size_t i = 0; i < b->groups.size(); ++i) {
+    auto& g = b->groups[i];
+    if (schedule == kScheduleBranches) g.branch = (int)i;
+    else if (schedule == kScheduleSerial) g.branch = -1;
+    else {
+      // groups are in descending order; each goes to the branch with fewer nodes so far
+      g.branch = (load[1] < load[0]) ? 1 : 0;
+      load[g.branch] += g.nodes;
+    }
+  }
+}
+
+// One stage of every solve under the batch's schedule (fork/join on events for the branches).
 hipError_t launch_stage(dg3d_batch* b, int stage, bool halo, int from) {
+  const size_t ng = b->groups.size();
+  std::vector<char> used(ng, 0);
+  for (const auto& g : b->groups)
+    if (g.branch >= 0) used[g.branch] = 1;
   hipError_t e = hipEventRecord(b->fork, b->stream);
+  for (size_t i = 0; i < ng && e == hipSuccess; ++i)
+    if (used[i]) e = hipStreamWaitEvent(b->groups[i].stream, b->fork, 0);
   for (auto& g : b->groups) {
     if (e != hipSuccess) break;
-    e = hipStreamWaitEvent(g.stream, b->fork, 0);
-    if (e == hipSuccess)
-      e = dg3d::launch_rhs_batch(stage_launch(b, g, g.variant, stage, halo, from), g.n, g.variant, g.grid[g.variant],
-                                 g.stream);
-    if (e == hipSuccess) e = hipEventRecord(g.done, g.stream);
-    if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, g.done, 0);
+    hipStream_t s = g.branch >= 0 ? b->groups[g.branch].stream : b->stream;
+    e = dg3d::launch_rhs_batch(stage_launch(b, g, g.variant, stage, halo, from), g.n, g.variant, g.grid[g.variant], s);
+  }
+  for (size_t i = 0; i < ng && e == hipSuccess; ++i) {
+    if (!used[i]) continue;
+    e = hipEventRecord(b->groups[i].done, b->groups[i].stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->groups[i].done, 0);
   }
   return e;
 }
@@ -210,6 +238,24 @@ int batch_autotune(dg3d_batch* b) {
     }
     g.variant = (int)(std::min_element(best, best + dg3d::kNumBatchVariants) - best);
   }
+  // then the schedule, on whole stages of the same launches
+  float sbest[kNumSchedules];
+  for (float& x : sbest) x = 1e30f;
+  for (int sweep = 0; sweep < 2; ++sweep) {
+    for (int sc = 0; sc < kNumSchedules; ++sc) {
+      set_schedule(b, sc);
+      e = launch_stage(b, 1, false, b->cur);
+      if (e == hipSuccess) e = hipEventRecord(b->ev0, b->stream);
+      for (int r = 0; r < 3 && e == hipSuccess; ++r) e = launch_stage(b, 1, false, b->cur);
+      if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
+      if (e != hipSuccess) return b->hip_fail(e, "batch autotune");
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, b->ev0, b->ev1);
+      sbest[sc] = std::min(sbest[sc], ms);
+    }
+  }
+  set_schedule(b, (int)(std::min_element(sbest, sbest + kNumSchedules) - sbest));
   b->tuned = true;
   return DG3D_OK;
 }
@@ -430,6 +476,7 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
     b->groups.push_back(g);
     if (e != hipSuccess) return bail(DG3D_E_HIP, std::string("batch streams: ") + hipGetErrorString(e));
   }
+  set_schedule(b, kScheduleBranches);
   e = hipDeviceSynchronize();
   if (e != hipSuccess) return bail(DG3D_E_HIP, std::string("batch setup: ") + hipGetErrorString(e));
   *out = b;
@@ -438,6 +485,20 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
 
 int dg3d_batch_destroy(dg3d_batch* b) {
   free_batch(b);
+  return DG3D_OK;
+}
+
+int dg3d_batch_set_schedule(dg3d_batch* b, int schedule) {
+  if (!b) return DG3D_E_INVALID;
+  if (schedule < 0 || schedule >= kNumSchedules) return b->fail(DG3D_E_INVALID, "batch: schedule out of range");
+  set_schedule(b, schedule);
+  drop_graphs(b);
+  return DG3D_OK;
+}
+
+int dg3d_batch_get_schedule(const dg3d_batch* b, int* schedule) {
+  if (!b || !schedule) return DG3D_E_INVALID;
+  *schedule = b->schedule;
   return DG3D_OK;
 }
 

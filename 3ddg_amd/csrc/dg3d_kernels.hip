@@ -419,6 +419,12 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   const unsigned mask = L.mask;
   const int E = (int)m.E;
   const int64_t fs = m.fs;
+  // the epilogue's scalars, read before the first global store: a launch record in global
+  // memory (the batched kernels) would otherwise be re-read, with vector loads, after every
+  // store that might alias it
+  const double ep_a = L.a, ep_b = L.b, ep_dt = L.dt;
+  double* const ep_out = L.out;
+  double* const ep_qout = L.q_out;
 
   // ---- phase 0: issue every global load of the workgroup -------------------------------
   // The node's state and RK register, the neighbour traces of the lane's face items, the
@@ -696,7 +702,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
       if (touch[f]) {
-        const Rsrc ro = make_rsrc(L.out + f * fs, fb64);
+        const Rsrc ro = make_rsrc(ep_out + f * fs, fb64);
         bstore<double>(bload<double>(ro, gi) + (double)rhs[f], ro, gi);
       }
     }
@@ -704,27 +710,27 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     if constexpr (std::is_same<DUT, double>::value) {
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
-        const double dn = L.a * pre[f] + L.dt * (double)rhs[f];
+        const double dn = ep_a * pre[f] + ep_dt * (double)rhs[f];
         bstore<double, kPolicyNT>(dn, rp[f], gi);
-        bstore<double, kPolicyNT>(q[f] + L.b * dn, make_rsrc(L.q_out + f * fs, fb64), gi);
+        bstore<double, kPolicyNT>(q[f] + ep_b * dn, make_rsrc(ep_qout + f * fs, fb64), gi);
       }
     } else {
-      const float af = (float)L.a, dtf = (float)L.dt;
+      const float af = (float)ep_a, dtf = (float)ep_dt;
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
         const float dn = af * (float)pre[f] + dtf * (float)rhs[f];
         bstore<float, kPolicyNT>(dn, rp[f], gi);
-        bstore<double, kPolicyNT>(q[f] + L.b * (double)dn, make_rsrc(L.q_out + f * fs, fb64), gi);
+        bstore<double, kPolicyNT>(q[f] + ep_b * (double)dn, make_rsrc(ep_qout + f * fs, fb64), gi);
       }
     }
   } else {
     // Grid.cpp:705, 765-766, 826-827: out = (rhs*(dt*beta) + in*beta) + old*alpha
-    const double sb = L.dt * L.b;
+    const double sb = ep_dt * ep_b;
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
-      double v = (double)rhs[f] * sb + q[f] * L.b;
-      if (L.a != 0.0) v += pre[f] * L.a;
-      bstore<double>(v, make_rsrc(L.out + f * fs, fb64), gi);
+      double v = (double)rhs[f] * sb + q[f] * ep_b;
+      if (ep_a != 0.0) v += pre[f] * ep_a;
+      bstore<double>(v, make_rsrc(ep_out + f * fs, fb64), gi);
     }
   }
 }

@@ -108,6 +108,8 @@ _SIGS = {
     "dg3d_batch_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "dg3d_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, _ip]),
     "dg3d_batch_set_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dg3d_batch_set_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dg3d_batch_get_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "dg3d_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "dg3d_batch_get_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp]),
     "dg3d_batch_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _ip, _ip, _ip, _ip, _ip]),
@@ -418,6 +420,14 @@ class Batch:
 
     def set_variant(self, variant):
         self._check(self._L.dg3d_batch_set_variant(self._h, int(variant)), "batch_set_variant")
+
+    def set_schedule(self, schedule):
+        self._check(self._L.dg3d_batch_set_schedule(self._h, int(schedule)), "batch_set_schedule")
+
+    def schedule(self):
+        v = ctypes.c_int()
+        self._check(self._L.dg3d_batch_get_schedule(self._h, ctypes.byref(v)), "batch_get_schedule")
+        return v.value
 
     def step(self, nsteps):
         self._check(self._L.dg3d_batch_step(self._h, int(nsteps)), "batch_step")

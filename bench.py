@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--xcd", type=int, default=1, help="workgroup order 0..3 when --variant is pinned")
     ap.add_argument("--batch-steps", type=int, default=20, help="timed steps of the batched workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-solve workload")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched 256-solve workload")
     ap.add_argument("--c5-steps", type=int, default=2000, help="steps of the combined 48^3 N=5 workload (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
@@ -207,6 +208,7 @@ def batch_run(pk, steps, warmup=1):
         b.step(steps)
         el = time.perf_counter() - t0
         info = b.info()
+        info["schedule"] = b.schedule()
         stage_ms, _ = b.last_step_timing()
         runs[name] = dict(b=b, elapsed=el, info=info, stage_ms=stage_ms)
     nodes = runs["mixed"]["info"]["total_nodes"]
@@ -229,7 +231,8 @@ def batch_run(pk, steps, warmup=1):
         fp64_hbm_frac=nodes * BYTES_FP64 / (d["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         mixed_rel_l2_max_vs_fp64=worst,
         orders=m["info"]["orders"], variants=m["info"]["variants"], tasks=m["info"]["tasks"],
-        grids=m["info"]["grids"],
+        grids=m["info"]["grids"], schedule=m["info"]["schedule"], fp64_variants=d["info"]["variants"],
+        fp64_schedule=d["info"]["schedule"],
     )
     return out
 
@@ -280,6 +283,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.batch_only:
+        print(json.dumps({"batch_c4": batch_run(pk, args.batch_steps)}))
+        return
     p = pk.problem.config_c2(num_elements=args.elements, steps=args.steps)
     ctx, r = timed_run(pk, p, pk.dg3d.FP64, args.steps, args.warmup, variant=args.variant, xcd=args.xcd)
     maxabs = ctx.reduce(pk.dg3d.REDUCE_MAXABS, 0)
