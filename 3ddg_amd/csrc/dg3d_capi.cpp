@@ -482,6 +482,15 @@ int dg3d_set_graphs(dg3d_ctx* c, int enable) {
   return DG3D_OK;
 }
 
+int dg3d_group_order(int order, const int* num_elements, int elems_per_group, int64_t groups, int64_t* out) {
+  if (!num_elements || !out || order < 0 || order >= dg3d::kNumOrders || elems_per_group < 1 || groups < 1 ||
+      groups >= (1ll << 31))
+    return DG3D_E_INVALID;
+  const dg3d::GroupMap g = dg3d::make_group_map(order, (uint32_t)groups, num_elements, elems_per_group);
+  for (int64_t w = 0; w < groups; ++w) out[w] = dg3d::map_group((uint32_t)w, g);
+  return DG3D_OK;
+}
+
 int dg3d_abi_version(void) { return DG3D_ABI_VERSION; }
 const char* dg3d_create_error(void) { return g_create_error.c_str(); }
 const char* dg3d_last_error(const dg3d_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }

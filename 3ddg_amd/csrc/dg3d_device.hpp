@@ -44,6 +44,91 @@ __host__ __device__ inline uint32_t div32(uint32_t n, Div32 d) {
   return (uint32_t)(((uint64_t)n * d.m) >> (31 + d.s));
 }
 
+// Workgroup -> element-group map of the stage kernels. The hardware dispatches workgroup w to
+// XCD w mod 8, and each XCD has its own L2, so the order decides which neighbour traces an XCD
+// finds in its L2:
+//   0: hardware order (XCD x takes groups x, x+8, ...);
+//   1: XCD x takes one contiguous slab of G/8 groups;
+//   2: XCD x takes runs of C consecutive groups (runs x, x+8, ... in time), with C one y-row of
+//      groups split over the 8 XCDs, so that the groups one row (y) and one plane (x) away fall
+//      on the same XCD while the 8 XCDs stay in one compact window of memory;
+//   3: order 1's slabs over a traversal tiled in blocks of TJ element rows (y) and TK groups (z),
+//      i outermost inside a block, then j, then the group: the groups one plane, one row and
+//      one element away are at most TJ*TK, TK and 1 positions apart. Falls back to order 1
+//      when the mesh does not divide.
+// The host prepares every divisor (G < 2^31 groups), so a workgroup spends a few scalar
+// instructions on it rather than 64-bit division sequences before its first load.
+constexpr int kTileJ = 8, kTileK = 8;
+struct GroupMap {
+  int mode;
+  uint32_t per, rem;          // orders 1, 3: G / 8, G % 8
+  uint32_t C, full;           // order 2: run length, groups covered by whole 8-run rounds
+  Div32 dC;
+  int tiled;                  // order 3: the mesh divides into tiles
+  uint32_t per_jb, per_kb, tt, tk, tj, kg, nE1;
+  Div32 d_jb, d_kb, d_tt, d_tk;
+};
+
+inline GroupMap make_group_map(int mode, uint32_t G, const int* nE, int epb) {
+  GroupMap g{};
+  g.mode = mode;
+  g.per = G / 8;
+  g.rem = G % 8;
+  if (mode == 2) {
+    uint32_t C = 4;
+    if (nE[2] % epb == 0) {
+      const uint32_t sy = (uint32_t)(nE[2] / epb);
+      C = (sy % 8 == 0) ? sy / 8 : sy;
+    }
+    g.C = C;
+    g.full = G / (8 * C) * (8 * C);
+    g.dC = make_div32(C);
+  }
+  if (mode == 3 && nE[2] % epb == 0) {
+    const uint32_t kg = (uint32_t)(nE[2] / epb);
+    const uint32_t tk = kg < (uint32_t)kTileK ? kg : kTileK;
+    const uint32_t tj = (uint32_t)nE[1] < (uint32_t)kTileJ ? (uint32_t)nE[1] : kTileJ;
+    if (kg % tk == 0 && nE[1] % tj == 0) {
+      g.tiled = 1;
+      g.kg = kg;
+      g.tk = tk;
+      g.tj = tj;
+      g.nE1 = (uint32_t)nE[1];
+      g.per_jb = tj * (uint32_t)nE[0] * kg;
+      g.per_kb = (uint32_t)nE[0] * tj * tk;
+      g.tt = tj * tk;
+      g.d_jb = make_div32(g.per_jb);
+      g.d_kb = make_div32(g.per_kb);
+      g.d_tt = make_div32(g.tt);
+      g.d_tk = make_div32(tk);
+    }
+  }
+  return g;
+}
+
+__host__ __device__ inline uint32_t map_group(uint32_t w, const GroupMap& g) {
+  if (g.mode == 1 || g.mode == 3) {
+    const uint32_t x = w & 7u, i = w >> 3;
+    const uint32_t s = x * g.per + (x < g.rem ? x : g.rem) + i;
+    if (g.mode == 1 || !g.tiled) return s;
+    const uint32_t jb = div32(s, g.d_jb);
+    uint32_t r = s - jb * g.per_jb;
+    const uint32_t kb = div32(r, g.d_kb);
+    r -= kb * g.per_kb;
+    const uint32_t ii = div32(r, g.d_tt);
+    r -= ii * g.tt;
+    const uint32_t jj = div32(r, g.d_tk), kk = r - jj * g.tk;
+    return (ii * g.nE1 + jb * g.tj + jj) * g.kg + kb * g.tk + kk;
+  }
+  if (g.mode == 2) {
+    if (w >= g.full) return w;
+    const uint32_t x = w & 7u, i = w >> 3;
+    const uint32_t q = div32(i, g.dC);
+    return q * (8 * g.C) + x * g.C + (i - q * g.C);
+  }
+  return w;
+}
+
 // Device layout of a field: element-major, with ES(n) node slots per element. n^3 is rounded
 // up to a multiple of 32 slots (n = 3, 5, 6, 7), so every element of an FP64 or FP32 field
 // starts on a 128-byte line and a workgroup's elements cover whole lines: no line is written
@@ -109,6 +194,10 @@ struct RhsLaunch {
 };
 
 hipError_t launch_rhs(const RhsLaunch& L, hipStream_t s);
+// The workgroup order of L.xcd_remap for a stage launch of `groups` element groups.
+inline GroupMap group_map_for(const RhsLaunch& L, int64_t groups, int epb) {
+  return make_group_map(L.xcd_remap, (uint32_t)groups, L.mesh.nE, epb);
+}
 constexpr int kNumVariants = 8;
 constexpr int kNumOrders = 4;
 

@@ -43,57 +43,6 @@ struct Variants {
   static constexpr int MINW1 = 6;
 };
 
-// Workgroup order. The hardware dispatches workgroup w to XCD w mod 8, and each XCD has its own
-// L2, so the order decides which neighbour traces an XCD finds in its L2:
-//   0: hardware order (XCD x takes groups x, x+8, ...);
-//   1: XCD x takes one contiguous slab of G/8 groups;
-//   2: XCD x takes runs of C consecutive groups (runs x, x+8, ... in time), with C chosen so
-//      that the groups one element row (y) and one element plane (x) away fall on the same XCD
-//      while the 8 XCDs stay in one compact window of memory.
-__device__ __forceinline__ int64_t remap_block(int64_t w, int64_t G, int mode, int64_t C) {
-  constexpr int64_t X = 8;
-  if (mode == 1) {
-    const int64_t per = G / X, rem = G % X, x = w % X, i = w / X;
-    return x * per + (x < rem ? x : rem) + i;
-  }
-  if (mode == 2) {
-    const int64_t full = G / (X * C) * (X * C);
-    if (w >= full) return w;
-    const int64_t x = w % X, i = w / X;
-    return (i / C) * (X * C) + x * C + (i % C);
-  }
-  return w;
-}
-// Order 3: order 1's XCD slabs over a tiled traversal of the element groups. With Kg = nE2/EPB
-// groups per (i, j) column, the traversal runs over blocks of TJ element rows (y) and TK groups
-// (z); inside a block i is outermost, then j, then the group: the groups one plane (x), one row
-// (y) and one element (z) away are at most TJ*TK, TK and 1 positions apart, so an XCD finds
-// most neighbour traces in its L2. Falls back to order 1 when the mesh does not divide.
-constexpr int kTileJ = 8, kTileK = 8;
-__device__ __forceinline__ int64_t tiled_group(int64_t s, int nE0, int nE1, int nE2, int epb) {
-  if (nE2 % epb != 0) return s;
-  const int kg = nE2 / epb;
-  const int tk = kg < kTileK ? kg : kTileK, tj = nE1 < kTileJ ? nE1 : kTileJ;
-  if (kg % tk != 0 || nE1 % tj != 0) return s;
-  const int64_t per_jb = (int64_t)tj * nE0 * kg, per_kb = (int64_t)nE0 * tj * tk;
-  const int64_t jb = s / per_jb;
-  int64_t r = s - jb * per_jb;
-  const int64_t kb = r / per_kb;
-  r -= kb * per_kb;
-  const int64_t i = r / (tj * tk);
-  r -= i * (tj * tk);
-  const int64_t jj = r / tk, kk = r - (r / tk) * tk;
-  return (i * nE1 + jb * tj + jj) * kg + kb * tk + kk;
-}
-
-// Run length of order 2 for EPB elements per group: a y-row of nE2/EPB groups split over 8 XCDs.
-__device__ __forceinline__ int64_t order_run(int nE2, int epb) {
-  if (nE2 % epb != 0) return 4;
-  const int sy = nE2 / epb;
-  if (sy % 8 == 0) return sy / 8;
-  return sy;
-}
-
 // kernelFuncs.cpp:17-20: P/rho0 = (rho' + rho'^2 * (B/A) * 0.5 / rho0) * c^2 / rho0
 template <typename CT>
 __device__ __forceinline__ CT momentum_f(CT rho, CT c, CT ba, CT ir0) {
@@ -737,15 +686,10 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
 
 template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT, int CONTRACT = 0, bool LATE = false,
           bool LDG = false>
-__global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunch L) {
+__global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunch L, const GroupMap gm) {
   __shared__ StageSmem<NN, EPB, CT, CONTRACT, LDG> sm;
-  const MeshArgs& m = L.mesh;
-  int64_t grp;
-  if (L.xcd_remap == 3)
-    grp = tiled_group(remap_block(blockIdx.x, gridDim.x, 1, 1), m.nE[0], m.nE[1], m.nE[2], EPB);
-  else
-    grp = remap_block(blockIdx.x, gridDim.x, L.xcd_remap, order_run(m.nE[2], EPB));
-  stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI, DUT, CONTRACT, LATE, LDG>(L, m, L.ext.trace, (int)(grp * EPB),
+  const uint32_t grp = map_group(blockIdx.x, gm);
+  stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI, DUT, CONTRACT, LATE, LDG>(L, L.mesh, L.ext.trace, (int)(grp * EPB),
                                                                            (int)threadIdx.x, sm);
 }
 
@@ -1080,12 +1024,13 @@ hipError_t launch_rhs_ldg(const RhsLaunch& L, hipStream_t s) {
   const int64_t blocks = (L.mesh.E + EPB - 1) / EPB;
   if (blocks <= 0) return hipSuccess;
   dim3 grid((unsigned)blocks), block(S::NT);
+  const GroupMap gm = group_map_for(L, blocks, EPB);
   if (L.epilogue == EPI_LSRK) {
-    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, 1, double, EPI_LSRK, double, 0, false, true>), grid, block, 0, s, L);
-    else hipLaunchKernelGGL((k_rhs<NN, EPB, 1, float, EPI_LSRK, float, 0, false, true>), grid, block, 0, s, L);
+    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, 1, double, EPI_LSRK, double, 0, false, true>), grid, block, 0, s, L, gm);
+    else hipLaunchKernelGGL((k_rhs<NN, EPB, 1, float, EPI_LSRK, float, 0, false, true>), grid, block, 0, s, L, gm);
   } else if (L.epilogue == EPI_RK33) {
-    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, 1, double, EPI_RK33, double, 0, false, true>), grid, block, 0, s, L);
-    else hipLaunchKernelGGL((k_rhs<NN, EPB, 1, float, EPI_RK33, double, 0, false, true>), grid, block, 0, s, L);
+    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, 1, double, EPI_RK33, double, 0, false, true>), grid, block, 0, s, L, gm);
+    else hipLaunchKernelGGL((k_rhs<NN, EPB, 1, float, EPI_RK33, double, 0, false, true>), grid, block, 0, s, L, gm);
   } else {
     return hipErrorInvalidValue;
   }
@@ -1098,19 +1043,20 @@ hipError_t launch_rhs_v(const RhsLaunch& L, hipStream_t s) {
   const int64_t blocks = (L.mesh.E + EPB - 1) / EPB;
   if (blocks <= 0) return hipSuccess;
   dim3 grid((unsigned)blocks), block(S::NT);
+  const GroupMap gm = group_map_for(L, blocks, EPB);
   if (L.epilogue == EPI_ACCUM) {
     if (L.mixed) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_ACCUM, double>), grid, block, 0, s, L);
+    hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_ACCUM, double>), grid, block, 0, s, L, gm);
   } else if (L.epilogue == EPI_LSRK) {
-    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_LSRK, double, 0, LATE>), grid, block, 0, s, L);
-    else if (L.contraction == 1) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float, 1>), grid, block, 0, s, L);
-    else if (L.contraction == 2) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float, 2>), grid, block, 0, s, L);
-    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float, 0, LATE>), grid, block, 0, s, L);
+    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_LSRK, double, 0, LATE>), grid, block, 0, s, L, gm);
+    else if (L.contraction == 1) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float, 1>), grid, block, 0, s, L, gm);
+    else if (L.contraction == 2) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float, 2>), grid, block, 0, s, L, gm);
+    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_LSRK, float, 0, LATE>), grid, block, 0, s, L, gm);
   } else {
-    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_RK33, double, 0, LATE>), grid, block, 0, s, L);
-    else if (L.contraction == 1) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double, 1>), grid, block, 0, s, L);
-    else if (L.contraction == 2) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double, 2>), grid, block, 0, s, L);
-    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double, 0, LATE>), grid, block, 0, s, L);
+    if (!L.mixed) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, double, EPI_RK33, double, 0, LATE>), grid, block, 0, s, L, gm);
+    else if (L.contraction == 1) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double, 1>), grid, block, 0, s, L, gm);
+    else if (L.contraction == 2) hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double, 2>), grid, block, 0, s, L, gm);
+    else hipLaunchKernelGGL((k_rhs<NN, EPB, MINW, float, EPI_RK33, double, 0, LATE>), grid, block, 0, s, L, gm);
   }
   return hipGetLastError();
 }

@@ -95,6 +95,8 @@ _SIGS = {
     "dg3d_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip]),
     "dg3d_set_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "dg3d_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dg3d_group_order": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int64,
+                                        ctypes.POINTER(ctypes.c_int64)]),
     "dg3d_reduce": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "dg3d_set_probes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
                                         ctypes.c_int]),
@@ -180,6 +182,17 @@ def basis_read_files(coeff_path, matrices_path, order):
     if rc:
         raise Dg3dError(rc, f"basis_read_files({coeff_path}, {matrices_path}, {order})")
     return x, w, M.reshape(n, n), S.reshape(n, n)
+
+
+def group_order(order, num_elements, elems_per_group, groups):
+    """Element group of every workgroup of a stage launch under a workgroup order (0..3)."""
+    nE = (ctypes.c_int * 3)(*num_elements)
+    out = np.zeros(int(groups), dtype=np.int64)
+    rc = lib().dg3d_group_order(int(order), nE, int(elems_per_group), int(groups),
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    if rc:
+        raise Dg3dError(rc, "group_order")
+    return out
 
 
 def sponge_sigma(num_elements, num_blocks, block_pos, elem_width, sponge_width, sigma_max, ramp):

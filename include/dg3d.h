@@ -175,6 +175,12 @@ int dg3d_stage(dg3d_ctx* ctx, int scheme, int stage, double dt, double t);
 int dg3d_autotune(dg3d_ctx* ctx, int scheme, int* variant, int* xcd_remap);
 int dg3d_set_variant(dg3d_ctx* ctx, int scheme, int variant, int xcd_remap);
 
+/* The workgroup order of the stage kernels (no counterpart in the reference): out[w] = element
+ * group of workgroup w, for a launch of `groups` groups of elems_per_group elements over a block
+ * of num_elements[3], under workgroup order `order` (0..3, as xcd_remap). A host function, for
+ * checking that every order visits every group once. */
+int dg3d_group_order(int order, const int* num_elements, int elems_per_group, int64_t groups, int64_t* out);
+
 /* CFL time step in FP64 on the device: dt = cfl / max_e( c(e) * sum_a 2 / (w_a(e) * dxi_min) ),
  * dxi_min the smallest LGL node gap on [-1,1] (the d-dimensional node-spacing CFL). */
 int dg3d_cfl_dt(dg3d_ctx* ctx, double cfl, double* dt);

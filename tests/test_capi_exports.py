@@ -36,3 +36,55 @@ def test_abi_version_and_create_error_without_device():
     assert L.dg3d_create(ctypes.byref(d), ctypes.byref(h)) == dg3d.E_UNSUPPORTED
     assert b"order" in L.dg3d_create_error()
     assert not h.value
+
+
+def _orders_by_definition(order, nE, epb, G):
+    """The workgroup orders written out with plain integer arithmetic (the stage kernels use
+    host-prepared magic divisors instead)."""
+    X = 8
+    out = []
+    for w in range(G):
+        if order in (1, 3):
+            per, rem, x, i = G // X, G % X, w % X, w // X
+            s = x * per + min(x, rem) + i
+            if order == 3 and nE[2] % epb == 0:
+                kg = nE[2] // epb
+                tk, tj = min(kg, 8), min(nE[1], 8)
+                if kg % tk == 0 and nE[1] % tj == 0:
+                    per_jb, per_kb = tj * nE[0] * kg, nE[0] * tj * tk
+                    jb, r = divmod(s, per_jb)
+                    kb, r = divmod(r, per_kb)
+                    ii, r = divmod(r, tj * tk)
+                    jj, kk = divmod(r, tk)
+                    s = (ii * nE[1] + jb * tj + jj) * kg + kb * tk + kk
+            out.append(s)
+        elif order == 2:
+            if nE[2] % epb != 0:
+                C = 4
+            else:
+                sy = nE[2] // epb
+                C = sy // 8 if sy % 8 == 0 else sy
+            full = G // (X * C) * (X * C)
+            if w >= full:
+                out.append(w)
+            else:
+                x, i = w % X, w // X
+                out.append((i // C) * (X * C) + x * C + i % C)
+        else:
+            out.append(w)
+    return out
+
+
+def test_workgroup_orders_are_permutations():
+    """Every workgroup order visits each element group exactly once, on meshes that divide into
+    the tiles and on meshes that do not, and equals its definition."""
+    cases = [((64, 64, 64), 2), ((40, 40, 40), 1), ((13, 7, 5), 1), ((9, 16, 24), 2), ((8, 8, 8), 4),
+             ((5, 3, 17), 4), ((1, 1, 1), 1), ((48, 48, 48), 4), ((3, 11, 16), 16), ((30, 9, 14), 2)]
+    for nE, epb in cases:
+        G = nE[0] * nE[1] * ((nE[2] + epb - 1) // epb) if nE[2] % epb == 0 else \
+            (nE[0] * nE[1] * nE[2] + epb - 1) // epb
+        for order in range(4):
+            got = dg3d.group_order(order, nE, epb, G)
+            assert sorted(got.tolist()) == list(range(G)), (nE, epb, order)
+            if G <= 70000:
+                assert got.tolist() == _orders_by_definition(order, nE, epb, G), (nE, epb, order)
