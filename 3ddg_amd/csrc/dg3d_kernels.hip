@@ -340,6 +340,8 @@ struct StageSmem {
   CT sLN[NN];
 };
 
+constexpr unsigned kItemExterior = 1u << 31;  // face item flag: zero exterior state
+
 // One workgroup's share of a stage: elements e0 .. e0+EPB-1 of the block described by m (field
 // arrays in L; traces: the 6 exterior face traces in memory, or null for none). NT is the
 // workgroup size (>= EPB * ES); every lane must call it, since it holds barriers. LATE issues the
@@ -432,9 +434,16 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   }
 
   // face items (element, direction, face node) of this lane; item = fle | dir << 6 | fn << 9 |
-  // self node << 16, or ~0u when the item has nothing to do
+  // self node << 16 | kItemExterior when the other side is the block's exterior without a trace
+  // (zero state), or ~0u when the item has nothing to do
   double nrho[R], nu[R], nq[R];
   unsigned item[R];
+  // the exterior trace pointers are the same for every lane: read them once, as scalars, and
+  // select per lane (a per-lane indexed read would be a vector load whose wait holds back the
+  // neighbour loads behind every load already in flight)
+  const double* trace_ptr[6];
+#pragma unroll
+  for (int d = 0; d < 6; ++d) trace_ptr[d] = traces ? traces[d] : nullptr;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int f = tid + r * NT;
@@ -457,25 +466,25 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
         int ec[3];
         elem_coords(m, (uint32_t)he, ec);
         const int nb = face_neighbour(m, he, ec, dir);
-        if (nb >= 0) {
-          const uint32_t oi = (uint32_t)nb * ES + on;
-          nrho[r] = bload<double>(rq[0], oi);
-          if (axis == 0) nu[r] = bload<double>(rq[1], oi);
-          else if (axis == 1) nu[r] = bload<double>(rq[2], oi);
-          else nu[r] = bload<double>(rq[3], oi);
-          if constexpr (LDG) {
-            if (axis == 0) nq[r] = bload<double>(rqq[0], oi);
-            else if (axis == 1) nq[r] = bload<double>(rqq[1], oi);
-            else nq[r] = bload<double>(rqq[2], oi);
-          }
-        } else {
-          const double* tr = traces ? traces[dir] : nullptr;
-          if (tr) {
-            const int64_t ti = ((int64_t)(-1 - nb) * NF2 + fn) * 3;
-            nrho[r] = tr[ti];
-            nu[r] = tr[ti + 1];
-            if constexpr (LDG) nq[r] = tr[ti + 2];
-          }
+        const double* tr = trace_ptr[0];
+#pragma unroll
+        for (int d = 1; d < 6; ++d)
+          if (dir == d) tr = trace_ptr[d];
+        // one address per value, chosen without a branch: the neighbour's node in the state, or
+        // the exterior trace; an exterior face without a trace reads the lane's own node and is
+        // zeroed in phase 2 (kItemExterior)
+        const bool inner = nb >= 0;
+        const bool state = inner || tr == nullptr;
+        item[r] |= (!inner && tr == nullptr) ? kItemExterior : 0u;
+        const int64_t oi = inner ? (int64_t)nb * ES + on : (int64_t)he * ES + sn;
+        const int64_t ti = ((int64_t)(-1 - nb) * NF2 + fn) * 3;
+        const double* prho = state ? L.q_in + oi : tr + ti;
+        const double* pu = state ? L.q_in + (1 + axis) * fs + oi : tr + ti + 1;
+        nrho[r] = *prho;
+        nu[r] = *pu;
+        if constexpr (LDG) {
+          const double* pq = state ? L.q_in + (4 + axis) * fs + oi : tr + ti + 2;
+          nq[r] = *pq;
         }
       }
     }
@@ -558,20 +567,27 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     const unsigned it = item[r];
     if (it == ~0u) continue;
     const int fle = it & 63u, dir = (it >> 6) & 7u, fn = (it >> 9) & 127u;
-    const int sn = it >> 16;
+    const int sn = (it >> 16) & 0x7FFFu;
     const int axis = 2 - (dir >> 1);
     const CT* F = sF + fle * 4 * NP;
     const CT mom_s = F[sn];
     const CT mass_s = F[(1 + axis) * NP + sn];
     const CT* K = sK + (fle * 6 + dir) * kNK;
-    const CT ro = (CT)nrho[r], uo = (CT)nu[r];
+    const bool zero = (it & kItemExterior) != 0u;  // exterior face without a trace
+    double vr = nrho[r], vu = nu[r];
+    // FP32 compute: an opaque use keeps the conversions here; folded back to the loads, they
+    // would make each wave wait for its neighbour loads as soon as they are issued
+    if constexpr (!std::is_same<CT, double>::value) asm volatile("" : "+v"(vr), "+v"(vu));
+    const CT ro = zero ? CT(0) : (CT)vr, uo = zero ? CT(0) : (CT)vu;
     const CT mom_o = (ro + ro * ro * K[8]) * K[9];
     const CT mass_o = uo * (ro * K[11] + K[10]);
     CT* Fl = sFl + ((fle * 6 + dir) * NF2 + fn) * 2;
     Fl[0] = K[0] * mom_o + K[1] * mom_s + K[2] * mass_o + K[3] * mass_s;
     Fl[1] = K[4] * mass_o + K[5] * mass_s + K[6] * mom_o + K[7] * mom_s;
     if constexpr (LDG) {
-      const CT qs = sm.sQ[(fle * 3 + axis) * NP + sn], qo = (CT)nq[r];
+      double vq = nq[r];
+      if constexpr (!std::is_same<CT, double>::value) asm volatile("" : "+v"(vq));
+      const CT qs = sm.sQ[(fle * 3 + axis) * NP + sn], qo = zero ? CT(0) : (CT)vq;
       sm.sFq[(fle * 6 + dir) * NF2 + fn] = (dir & 1) ? CT(-0.5) * (qs + qo) : CT(0.5) * (qo + qs);
     }
   }
