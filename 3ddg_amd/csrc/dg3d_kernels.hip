@@ -433,6 +433,45 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     dcoef = (CT)bload<double>(rm, mi + MAT_DIFF);
   }
 
+  // helper lanes, one per (element, face): material of both sides and 1/width. These and the
+  // line operators are issued before the neighbour gathers: phase 1 needs them, and the wait
+  // for them before the first barrier then leaves the gathers in flight until phase 2
+  const bool helper = tid < EPB * 6;
+  const int hle = tid / 6, hdir = tid - (tid / 6) * 6;
+  const int he = e0 + hle;
+  double ms_c = 1.0, ms_r = 1.0, ms_ir = 1.0, mo_c = 1.0, mo_r = 1.0, mo_ba = 0.0, mo_ir = 1.0, hiw = 0.0;
+  if (helper && he < E) {
+    int ec[3];
+    elem_coords(m, (uint32_t)he, ec);
+    const int nb = face_neighbour(m, he, ec, hdir);
+    const uint32_t ms = (uint32_t)he * kMatStride;
+    // the halo element has the adjacent element's material (Block.cpp:167-248)
+    const uint32_t mo = (uint32_t)(nb >= 0 ? nb : he) * kMatStride;
+    ms_c = bload<double>(rm, ms + MAT_C);
+    ms_r = bload<double>(rm, ms + MAT_RHO0);
+    ms_ir = bload<double>(rm, ms + MAT_IRHO0);
+    mo_c = bload<double>(rm, mo + MAT_C);
+    mo_r = bload<double>(rm, mo + MAT_RHO0);
+    mo_ba = bload<double>(rm, mo + MAT_BA);
+    mo_ir = bload<double>(rm, mo + MAT_IRHO0);
+    const int axis = 2 - (hdir >> 1);
+    if ((hdir & 1) == 0) hiw = sel3(axis, m.inv_w[0], m.inv_w[1], m.inv_w[2])[sel3(axis, ec[0], ec[1], ec[2])];
+  }
+  CT av = CT(0), l0v = CT(0), lnv = CT(0);
+  if (tid < NN * NN) {
+    if constexpr (std::is_same<CT, double>::value) av = L.opd[0][tid];
+    else av = L.opf[0][tid];
+  }
+  if (tid < NN) {
+    if constexpr (std::is_same<CT, double>::value) {
+      l0v = L.opd[1][tid];
+      lnv = L.opd[2][tid];
+    } else {
+      l0v = L.opf[1][tid];
+      lnv = L.opf[2][tid];
+    }
+  }
+
   // face items (element, direction, face node) of this lane; item = fle | dir << 6 | fn << 9 |
   // self node << 16 | kItemExterior when the other side is the block's exterior without a trace
   // (zero state), or ~0u when the item has nothing to do
@@ -487,43 +526,6 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
           nq[r] = *pq;
         }
       }
-    }
-  }
-
-  // helper lanes, one per (element, face): material of both sides and 1/width
-  const bool helper = tid < EPB * 6;
-  const int hle = tid / 6, hdir = tid - (tid / 6) * 6;
-  const int he = e0 + hle;
-  double ms_c = 1.0, ms_r = 1.0, ms_ir = 1.0, mo_c = 1.0, mo_r = 1.0, mo_ba = 0.0, mo_ir = 1.0, hiw = 0.0;
-  if (helper && he < E) {
-    int ec[3];
-    elem_coords(m, (uint32_t)he, ec);
-    const int nb = face_neighbour(m, he, ec, hdir);
-    const uint32_t ms = (uint32_t)he * kMatStride;
-    // the halo element has the adjacent element's material (Block.cpp:167-248)
-    const uint32_t mo = (uint32_t)(nb >= 0 ? nb : he) * kMatStride;
-    ms_c = bload<double>(rm, ms + MAT_C);
-    ms_r = bload<double>(rm, ms + MAT_RHO0);
-    ms_ir = bload<double>(rm, ms + MAT_IRHO0);
-    mo_c = bload<double>(rm, mo + MAT_C);
-    mo_r = bload<double>(rm, mo + MAT_RHO0);
-    mo_ba = bload<double>(rm, mo + MAT_BA);
-    mo_ir = bload<double>(rm, mo + MAT_IRHO0);
-    const int axis = 2 - (hdir >> 1);
-    if ((hdir & 1) == 0) hiw = sel3(axis, m.inv_w[0], m.inv_w[1], m.inv_w[2])[sel3(axis, ec[0], ec[1], ec[2])];
-  }
-  CT av = CT(0), l0v = CT(0), lnv = CT(0);
-  if (tid < NN * NN) {
-    if constexpr (std::is_same<CT, double>::value) av = L.opd[0][tid];
-    else av = L.opf[0][tid];
-  }
-  if (tid < NN) {
-    if constexpr (std::is_same<CT, double>::value) {
-      l0v = L.opd[1][tid];
-      lnv = L.opd[2][tid];
-    } else {
-      l0v = L.opf[1][tid];
-      lnv = L.opf[2][tid];
     }
   }
 
