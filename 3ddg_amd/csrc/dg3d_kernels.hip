@@ -43,16 +43,23 @@ struct Variants {
   static constexpr int MINW1 = 6;
 };
 
-// kernelFuncs.cpp:17-20: P/rho0 = (rho' + rho'^2 * (B/A) * 0.5 / rho0) * c^2 / rho0
+// kernelFuncs.cpp:17-20: P/rho0 = (rho' + rho'^2 * (B/A) * 0.5 / rho0) * c^2 / rho0, written
+// with the element constants k8 = (B/A) * 0.5 / rho0 and k9 = c^2 / rho0.
+// kernelFuncs.cpp:26-30: rho0 u if B/A == 0 exactly, else u (rho' + rho0), written as
+// u (rho' k11 + k10) with k10 = rho0 and k11 = 0 or 1 (the same value for finite rho').
+// The stage evaluates both through these functions for its own nodes (phase 1) and for gathered
+// neighbour traces (phase 2), so that a neighbour read from the workgroup's LDS and one gathered
+// from HBM give the same bits.
 template <typename CT>
-__device__ __forceinline__ CT momentum_f(CT rho, CT c, CT ba, CT ir0) {
-  return (rho + rho * rho * ba * CT(0.5) * ir0) * (c * c * ir0);
-}
-// kernelFuncs.cpp:26-30: rho0 u if B/A == 0 exactly, else u (rho' + rho0)
+__device__ __forceinline__ CT mom_k8(CT ba, CT ir0) { return ba * CT(0.5) * ir0; }
 template <typename CT>
-__device__ __forceinline__ CT mass_f(CT u, CT rho, CT r0, CT ba) {
-  return (ba == CT(0)) ? u * r0 : u * (rho + r0);
-}
+__device__ __forceinline__ CT mom_k9(CT c, CT ir0) { return c * c * ir0; }
+template <typename CT>
+__device__ __forceinline__ CT mass_k11(CT ba) { return (ba == CT(0)) ? CT(0) : CT(1); }
+template <typename CT>
+__device__ __forceinline__ CT momentum_k(CT rho, CT k8, CT k9) { return (rho + rho * rho * k8) * k9; }
+template <typename CT>
+__device__ __forceinline__ CT mass_k(CT u, CT rho, CT k10, CT k11) { return u * (rho * k11 + k10); }
 
 __device__ __forceinline__ double source_value(const ExtArgs& x, double t) {
   const double a = (t - x.t0) / x.tau;
@@ -154,10 +161,10 @@ __device__ __forceinline__ void face_coeffs(int side, CT c1, CT r1, CT ir1, CT c
   K[5] = sg * c1 * s1;
   K[6] = -iz * r2 * s1;
   K[7] = iz * r1 * s1;
-  K[8] = ba2 * CT(0.5) * ir2;
-  K[9] = c2 * c2 * ir2;
+  K[8] = mom_k8(ba2, ir2);
+  K[9] = mom_k9(c2, ir2);
   K[10] = r2;
-  K[11] = (ba2 == CT(0)) ? CT(0) : CT(1);
+  K[11] = mass_k11(ba2);
 }
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -341,6 +348,7 @@ struct StageSmem {
 };
 
 constexpr unsigned kItemExterior = 1u << 31;  // face item flag: zero exterior state
+constexpr unsigned kItemLocal = 1u << 30;     // face item flag: the neighbour is in the workgroup
 
 // One workgroup's share of a stage: elements e0 .. e0+EPB-1 of the block described by m (field
 // arrays in L; traces: the 6 exterior face traces in memory, or null for none). NT is the
@@ -357,6 +365,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   using S = Geo<NN, EPB>;
   constexpr int NP = S::NP, NF2 = S::NF2;
   static_assert(NT >= EPB * S::ES, "workgroup too small for the element group");
+  static_assert(EPB <= 32 && NN * NN * NN <= 512, "face item encoding: local element in 5 bits, node in 9");
   constexpr int NITEMS = EPB * 6 * NF2;            // face work items of the workgroup
   constexpr int R = (NITEMS + NT - 1) / NT;        // face items per lane
   CT* const sF = sm.sF;
@@ -474,7 +483,8 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
 
   // face items (element, direction, face node) of this lane; item = fle | dir << 6 | fn << 9 |
   // self node << 16 | kItemExterior when the other side is the block's exterior without a trace
-  // (zero state), or ~0u when the item has nothing to do
+  // (zero state) | kItemLocal | local element << 25 when the neighbour element is one of this
+  // workgroup's, or ~0u when the item has nothing to do
   double nrho[R], nu[R], nq[R];
   unsigned item[R];
   // the exterior trace pointers are the same for every lane: read them once, as scalars, and
@@ -515,15 +525,20 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
         const bool inner = nb >= 0;
         const bool state = inner || tr == nullptr;
         item[r] |= (!inner && tr == nullptr) ? kItemExterior : 0u;
-        const int64_t oi = inner ? (int64_t)nb * ES + on : (int64_t)he * ES + sn;
-        const int64_t ti = ((int64_t)(-1 - nb) * NF2 + fn) * 3;
-        const double* prho = state ? L.q_in + oi : tr + ti;
-        const double* pu = state ? L.q_in + (1 + axis) * fs + oi : tr + ti + 1;
-        nrho[r] = *prho;
-        nu[r] = *pu;
-        if constexpr (LDG) {
-          const double* pq = state ? L.q_in + (4 + axis) * fs + oi : tr + ti + 2;
-          nq[r] = *pq;
+        // a neighbour in this workgroup: phase 2 reads its kernel functions from LDS, no load
+        const bool local = !LDG && inner && nb >= e0 && nb < e0 + EPB;
+        item[r] |= local ? (kItemLocal | ((unsigned)(nb - e0) << 25)) : 0u;
+        if (!local) {
+          const int64_t oi = inner ? (int64_t)nb * ES + on : (int64_t)he * ES + sn;
+          const int64_t ti = ((int64_t)(-1 - nb) * NF2 + fn) * 3;
+          const double* prho = state ? L.q_in + oi : tr + ti;
+          const double* pu = state ? L.q_in + (1 + axis) * fs + oi : tr + ti + 1;
+          nrho[r] = *prho;
+          nu[r] = *pu;
+          if constexpr (LDG) {
+            const double* pq = state ? L.q_in + (4 + axis) * fs + oi : tr + ti + 2;
+            nq[r] = *pq;
+          }
         }
       }
     }
@@ -533,9 +548,10 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   if (valid) {
     const CT rho = (CT)q[0];
     CT* F = sF + le * 4 * NP;
-    F[node] = momentum_f(rho, c, ba, ir0);
+    F[node] = momentum_k(rho, mom_k8(ba, ir0), mom_k9(c, ir0));
+    const CT k11 = mass_k11(ba);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) F[(1 + a) * NP + node] = mass_f((CT)q[1 + a], rho, r0, ba);
+    for (int a = 0; a < 3; ++a) F[(1 + a) * NP + node] = mass_k((CT)q[1 + a], rho, r0, k11);
   }
   if constexpr (LDG) {
     if (valid) {
@@ -569,7 +585,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     const unsigned it = item[r];
     if (it == ~0u) continue;
     const int fle = it & 63u, dir = (it >> 6) & 7u, fn = (it >> 9) & 127u;
-    const int sn = (it >> 16) & 0x7FFFu;
+    const int sn = (it >> 16) & 0x1FFu;
     const int axis = 2 - (dir >> 1);
     const CT* F = sF + fle * 4 * NP;
     const CT mom_s = F[sn];
@@ -581,8 +597,16 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     // would make each wave wait for its neighbour loads as soon as they are issued
     if constexpr (!std::is_same<CT, double>::value) asm volatile("" : "+v"(vr), "+v"(vu));
     const CT ro = zero ? CT(0) : (CT)vr, uo = zero ? CT(0) : (CT)vu;
-    const CT mom_o = (ro + ro * ro * K[8]) * K[9];
-    const CT mass_o = uo * (ro * K[11] + K[10]);
+    CT mom_o = momentum_k(ro, K[8], K[9]);
+    CT mass_o = mass_k(uo, ro, K[10], K[11]);
+    if (it & kItemLocal) {
+      // the neighbour element is in this workgroup: phase 1 computed the same values
+      const int b = fn / NN, cc = fn - (fn / NN) * NN;
+      const int on = face_node<NN>(axis, (dir & 1) ? 0 : NN - 1, b, cc);
+      const CT* Fo = sF + ((it >> 25) & 31u) * 4 * NP;
+      mom_o = Fo[on];
+      mass_o = Fo[(1 + axis) * NP + on];
+    }
     CT* Fl = sFl + ((fle * 6 + dir) * NF2 + fn) * 2;
     Fl[0] = K[0] * mom_o + K[1] * mom_s + K[2] * mass_o + K[3] * mass_s;
     Fl[1] = K[4] * mass_o + K[5] * mass_s + K[6] * mom_o + K[7] * mom_s;
