@@ -88,3 +88,47 @@ def test_workgroup_orders_are_permutations():
             assert sorted(got.tolist()) == list(range(G)), (nE, epb, order)
             if G <= 70000:
                 assert got.tolist() == _orders_by_definition(order, nE, epb, G), (nE, epb, order)
+
+
+def test_create_rejects_bad_descriptors_on_the_host():
+    """Descriptors the library cannot run are rejected before any device call: 2^29 nodes or more
+    in one block (32-bit offsets within a field), a bad contraction or precision, a missing
+    operator, an empty axis."""
+    import numpy as np
+    L = dg3d.lib()
+    n = 8
+    arrs = [np.zeros(n * n), np.zeros(n * n), np.zeros(n), np.zeros(n)]
+
+    def desc(order=7, nE=(8, 8, 8), precision=dg3d.FP64, contraction=0, regs=1, missing=False):
+        d = dg3d.Desc()
+        d.order = order
+        for a in range(3):
+            d.num_elements[a] = nE[a]
+        d.precision = precision
+        d.contraction = contraction
+        d.num_registers = regs
+        ptrs = [a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in arrs]
+        d.mass, d.stiff, d.lgl_nodes, d.lgl_weights = ptrs
+        if missing:
+            d.stiff = ctypes.POINTER(ctypes.c_double)()
+        return d
+
+    def create(d):
+        h = ctypes.c_void_p()
+        rc = L.dg3d_create(ctypes.byref(d), ctypes.byref(h))
+        assert not h.value
+        return rc, L.dg3d_create_error()
+
+    # 102^3 elements at N=7 take 512 slots each: 5.4e8 >= 2^29 nodes
+    rc, msg = create(desc(nE=(102, 102, 102)))
+    assert rc == dg3d.E_INVALID and b"2^29" in msg
+    rc, msg = create(desc(contraction=7))
+    assert rc == dg3d.E_INVALID and b"contraction" in msg
+    rc, msg = create(desc(precision=5))
+    assert rc == dg3d.E_INVALID and b"precision" in msg
+    rc, msg = create(desc(missing=True))
+    assert rc == dg3d.E_INVALID and b"required" in msg
+    rc, msg = create(desc(nE=(4, 0, 4)))
+    assert rc == dg3d.E_INVALID and b"num_elements" in msg
+    rc, msg = create(desc(regs=0))
+    assert rc == dg3d.E_INVALID and b"num_registers" in msg
