@@ -63,3 +63,53 @@ def test_c3_full_size_mixed_vs_fp64():
     print(f"  rel L2 per field {err}", flush=True)
     assert np.linalg.norm(out[0] - st) > 0
     assert 0.0 < err.max() < 1e-5, err
+
+
+def test_c5_full_size_two_steps_vs_oracle():
+    """48^3 N=5, z stretched by 1.02 per element, two layers, focused burst with per-node delays."""
+    print("c5_full_size_two_steps_vs_oracle", flush=True)
+    p = problem.config_c5(num_elements=48, steps=2)
+    st = _random_state(p, 7, 0.4, 1e-3)
+    dt = problem.cfl_dt_host(p)
+    ctx = problem.make_context(p)
+    ctx.set_state(0, st)
+    ctx.step(dg3d.LSRK54, dt, 2, 0.0)
+    g = ctx.get_state(0)
+    ctx.close()
+    o = Oracle(p, literal=False, num_registers=3)
+    load_problem_into(o, p, 0)
+    o.set_state(0, st)
+    o.set_time(0.0)
+    for _ in range(2):
+        o.lsrk54(dt, 0, 1, 2)
+    r = o.get_state(0)
+    del o
+    err = rel_l2(g, r)
+    print(f"  rel L2 per field {err}", flush=True)
+    assert err.max() < 1e-12, err
+
+
+def test_c4_full_batch_vs_oracle():
+    """The whole 256-solve batch in FP64 (5.26e7 nodes) for three steps; the solves of every
+    order that come first in the draw are checked against the oracle."""
+    print("c4_full_batch_vs_oracle", flush=True)
+    ps = problem.config_c4(steps=3)
+    b = problem.make_batch(ps, precision=dg3d.FP64)
+    b.step(3)
+    picked = {}
+    for i, p in enumerate(ps):
+        picked.setdefault(p.order, i)
+    for order, i in sorted(picked.items()):
+        p = ps[i]
+        o = Oracle(p, literal=False, num_registers=3)
+        load_problem_into(o, p, 0)
+        o.set_time(0.0)
+        dt = problem.cfl_dt_host(p)
+        for _ in range(3):
+            o.lsrk54(dt, 0, 1, 2)
+        r = o.get_state(0)
+        err = rel_l2(b.get_state(i), r)
+        print(f"  solve {i} order {order} {p.num_elements[0]}^3: {err.max():.2e}", flush=True)
+        assert np.linalg.norm(r) > 0
+        assert err.max() < 1e-12, (i, err)
+    b.close()
