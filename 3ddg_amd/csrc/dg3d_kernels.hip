@@ -617,11 +617,10 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
       sm.sFq[(fle * 6 + dir) * NF2 + fn] = (dir & 1) ? CT(-0.5) * (qs + qo) : CT(0.5) * (qo + qs);
     }
   }
+  // the matrix-core contraction reads only phase 1's LDS (sF, sA) and writes its own array, so
+  // it runs beside the flux pairs, before the barrier that phase 3 needs anyway
+  if constexpr (CONTRACT != 0) contract_mfma<NN, EPB, NT, CONTRACT>(sF, sA, sOut, tid);
   __syncthreads();
-  if constexpr (CONTRACT != 0) {
-    contract_mfma<NN, EPB, NT, CONTRACT>(sF, sA, sOut, tid);
-    __syncthreads();
-  }
 
   if (!io) return;
 
