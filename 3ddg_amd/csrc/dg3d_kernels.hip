@@ -170,23 +170,16 @@ __device__ __forceinline__ void face_coeffs(int side, CT c1, CT r1, CT ir1, CT c
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 
-// float -> bf16 bits, round to nearest even (finite values), and back
-__device__ __forceinline__ unsigned short f2bf(float x) {
-  unsigned u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
-// x = hi + mid + lo as three bf16 pieces; both residuals are exact in FP32 and the
-// representation error is below 2^-26 |x|.
+// x = hi + mid + lo as three bf16 pieces (hardware conversion, round to nearest even); both
+// residuals are exact in FP32 and the representation error is below 2^-26 |x|.
 __device__ __forceinline__ void split3(float x, short& h, short& m, short& l) {
-  const unsigned short a = f2bf(x);
-  const float r1 = x - bf2f(a);
-  const unsigned short b = f2bf(r1);
-  const float r2 = r1 - bf2f(b);
-  h = (short)a;
-  m = (short)b;
-  l = (short)f2bf(r2);
+  const __bf16 a = (__bf16)x;
+  const float r1 = x - (float)a;
+  const __bf16 b = (__bf16)r1;
+  const float r2 = r1 - (float)b;
+  h = __builtin_bit_cast(short, a);
+  m = __builtin_bit_cast(short, b);
+  l = __builtin_bit_cast(short, (__bf16)r2);
 }
 
 // Line of a contraction column: a column is (element, axis, field, face node) with the field
@@ -206,6 +199,49 @@ __device__ __forceinline__ void column_line(int col, int& fo, int& oo, int& ls) 
   fo = le * 4 * NP + (f == 0 ? 0 : (1 + a)) * NP + base;
   oo = (le * 6 + af) * NP + base;
 }
+
+// The P = 16/NN columns a lane touches in one tile, one per diagonal block (columns col0 +
+// 16 b), mapped to their lines once per tile. Row k of the tile belongs to block k / NN.
+template <int NN>
+struct TileColumns {
+  static constexpr int P = 16 / NN;
+  int fo[P], oo[P], ls[P];
+  bool ok[P];
+  __device__ __forceinline__ TileColumns(int col0, int C) {
+#pragma unroll
+    for (int b = 0; b < P; ++b) {
+      const int col = col0 + 16 * b;
+      ok[b] = col < C;
+      fo[b] = oo[b] = ls[b] = 0;
+      if (ok[b]) column_line<NN>(col, fo[b], oo[b], ls[b]);
+    }
+  }
+  // the line of row k: one LDS access, its offset selected among the P columns
+  __device__ __forceinline__ bool pick(int k, const int* base, int& off) const {
+    const int b = k / NN, m = k - b * NN;
+    int o = 0, st = 0;
+    bool in = false;
+#pragma unroll
+    for (int bb = 0; bb < P; ++bb) {
+      o = (b == bb) ? base[bb] : o;
+      st = (b == bb) ? ls[bb] : st;
+      in = (b == bb) ? ok[bb] : in;
+    }
+    off = o + m * st;
+    return in;
+  }
+  // operand of row k (0 outside the blocks and past the last column)
+  __device__ __forceinline__ float load(const float* sF, int k) const {
+    int off;
+    const bool in = pick(k, fo, off);
+    const float v = sF[in ? off : 0];
+    return in ? v : 0.f;
+  }
+  __device__ __forceinline__ void store(float* sOut, int i, float v) const {
+    int off;
+    if (pick(i, oo, off)) sOut[off] = v;
+  }
+};
 
 // The workgroup's derivative contractions as one GEMM on the matrix cores:
 // out[i][col] = sum_m A[i][m] line_col[m], one column per (element, axis, field, face node).
@@ -238,29 +274,15 @@ __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf,
       aop[s] = in ? sAf[(j % NN) * NN + (k % NN)] : 0.f;
     }
     for (int t = wave; t < T; t += W) {
+      TileColumns<NN> tc(t * TC + j, C);
       floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int s = 0; s < KS; ++s) {
         const int k = 4 * s + g;
-        const int col = t * TC + (k / NN) * 16 + j;
-        float bv = 0.f;
-        if (k < P * NN && col < C) {
-          int fo, oo, ls;
-          column_line<NN>(col, fo, oo, ls);
-          bv = sF[fo + (k % NN) * ls];
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aop[s], bv, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aop[s], tc.load(sF, k), acc, 0, 0, 0);
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int ip = 4 * g + r;
-        const int col = t * TC + (ip / NN) * 16 + j;
-        if (ip / NN < P && col < C) {
-          int fo, oo, ls;
-          column_line<NN>(col, fo, oo, ls);
-          sOut[oo + (ip % NN) * ls] = acc[r];
-        }
-      }
+      for (int r = 0; r < 4; ++r) tc.store(sOut, 4 * g + r, acc[r]);
     }
   } else {
     bf16x4 ah, am, al;
@@ -275,17 +297,11 @@ __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf,
       al[q] = l;
     }
     for (int t = wave; t < T; t += W) {
+      TileColumns<NN> tc(t * TC + j, C);
       bf16x4 bh, bm, bl;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int k = 4 * g + q;
-        const int col = t * TC + (k / NN) * 16 + j;
-        float bv = 0.f;
-        if (k < P * NN && col < C) {
-          int fo, oo, ls;
-          column_line<NN>(col, fo, oo, ls);
-          bv = sF[fo + (k % NN) * ls];
-        }
+        const float bv = tc.load(sF, 4 * g + q);
         short h, m, l;
         split3(bv, h, m, l);
         bh[q] = h;
@@ -300,15 +316,7 @@ __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf,
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bh, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int ip = 4 * g + r;
-        const int col = t * TC + (ip / NN) * 16 + j;
-        if (ip / NN < P && col < C) {
-          int fo, oo, ls;
-          column_line<NN>(col, fo, oo, ls);
-          sOut[oo + (ip % NN) * ls] = acc[r];
-        }
-      }
+      for (int r = 0; r < 4; ++r) tc.store(sOut, 4 * g + r, acc[r]);
     }
   }
 }

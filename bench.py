@@ -44,9 +44,10 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=20, help="timed steps of the batched workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-solve workload")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched 256-solve workload")
-    ap.add_argument("--case", choices=["c3m_mfma_f32", "c3m_mfma_bf16", "c5m_mfma_bf16", "c5_fp64"], default=None,
-                    help="time one workload alone (for profiling): 64^3 N=7 or 48^3 N=5, mixed with a matrix-core "
-                         "contraction, or 48^3 N=5 FP64; prints one JSON line")
+    ap.add_argument("--case", choices=["c3m_valu", "c3m_mfma_f32", "c3m_mfma_bf16", "c5m_valu", "c5m_mfma_f32",
+                                       "c5m_mfma_bf16", "c5_fp64"], default=None,
+                    help="time one workload alone (for profiling): 64^3 N=7 or 48^3 N=5, mixed with the VALU or a "
+                         "matrix-core contraction, or 48^3 N=5 FP64; prints one JSON line")
     ap.add_argument("--c5-steps", type=int, default=2000, help="steps of the combined 48^3 N=5 workload (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
@@ -296,7 +297,8 @@ def main():
             prec, con, bpn = pk.dg3d.FP64, 0, BYTES_FP64
         else:
             prec, bpn = pk.dg3d.MIXED, BYTES_MIXED
-            con = pk.dg3d.CONTRACT_MFMA_F32 if rest == "mfma_f32" else pk.dg3d.CONTRACT_MFMA_BF16
+            con = {"valu": pk.dg3d.CONTRACT_VALU, "mfma_f32": pk.dg3d.CONTRACT_MFMA_F32,
+                   "mfma_bf16": pk.dg3d.CONTRACT_MFMA_BF16}[rest]
         ctx, r = timed_run(pk, p, prec, args.steps, args.warmup, variant=args.variant, xcd=args.xcd, contraction=con)
         ctx.close()
         print(json.dumps(dict(case=args.case, elements=list(p.num_elements), order=p.order, dof=p.dof,
