@@ -93,35 +93,35 @@ namespace {
 
 void drop_graphs(dg3d_batch* b) {
   for (auto& g : b->graph) {
-    if (g) hipGraphExecDestroy(g);
+    if (g) (void)hipGraphExecDestroy(g);
     g = nullptr;
   }
 }
 
 void free_batch(dg3d_batch* b) {
   if (!b) return;
-  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graphs(b);
-  for (double* p : b->allocs) hipFree(p);
+  for (double* p : b->allocs) (void)hipFree(p);
   for (auto& kv : b->ops) {
     for (int i = 0; i < 3; ++i) {
-      hipFree(kv.second.opd[i]);
-      hipFree(kv.second.opf[i]);
+      (void)hipFree(kv.second.opd[i]);
+      (void)hipFree(kv.second.opf[i]);
     }
   }
-  hipFree(b->arena[0]);
-  hipFree(b->arena[1]);
-  hipFree(b->du);
-  hipFree(b->d_records);
+  (void)hipFree(b->arena[0]);
+  (void)hipFree(b->arena[1]);
+  (void)hipFree(b->du);
+  (void)hipFree(b->d_records);
   for (auto& g : b->groups) {
-    for (auto* t : g.d_tasks) hipFree(t);
-    if (g.done) hipEventDestroy(g.done);
-    if (g.stream) hipStreamDestroy(g.stream);
+    for (auto* t : g.d_tasks) (void)hipFree(t);
+    if (g.done) (void)hipEventDestroy(g.done);
+    if (g.stream) (void)hipStreamDestroy(g.stream);
   }
-  if (b->fork) hipEventDestroy(b->fork);
-  if (b->ev0) hipEventDestroy(b->ev0);
-  if (b->ev1) hipEventDestroy(b->ev1);
-  if (b->stream) hipStreamDestroy(b->stream);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
 
@@ -139,7 +139,7 @@ dg3d::BatchLaunch stage_launch(const dg3d_batch* b, const dg3d_batch::Group& g, 
 // Stage schedules: every order on a branch of its own (the orders share the device); all orders
 // one after another on the batch stream, largest first (each has the device to itself); or two
 // branches with about the same number of nodes.
-constexpr int kScheduleBranches = 0, kScheduleSerial = 1, kScheduleTwo = 2, kNumSchedules = 3;
+constexpr int kScheduleBranches = 0, kScheduleSerial = 1, kNumSchedules = 3;  // 2: two balanced branches
 
 void set_schedule(dg3d_batch* b, int schedule) {
   b->schedule = schedule;
@@ -148,7 +148,7 @@ void set_schedule(dg3d_batch* b, int schedule) {
     auto& g = b->groups[i];
     if (schedule == kScheduleBranches) g.branch = (int)i;
     else if (schedule == kScheduleSerial) g.branch = -1;
-    else {
+    else {  // kScheduleTwo
       // groups are in descending order; each goes to the branch with fewer nodes so far
       g.branch = (load[1] < load[0]) ? 1 : 0;
       load[g.branch] += g.nodes;
@@ -198,12 +198,12 @@ hipError_t build_graphs(dg3d_batch* b) {
     hipError_t le = launch_step(b, from, false);
     e = hipStreamEndCapture(b->stream, &g);
     if (le != hipSuccess) {
-      if (g) hipGraphDestroy(g);
+      if (g) (void)hipGraphDestroy(g);
       return le;
     }
     if (e != hipSuccess) return e;
     e = hipGraphInstantiate(&b->graph[from], g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -232,7 +232,7 @@ int batch_autotune(dg3d_batch* b) {
         if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
         if (e != hipSuccess) return b->hip_fail(e, "batch autotune");
         float ms = 0.f;
-        hipEventElapsedTime(&ms, b->ev0, b->ev1);
+        (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
         best[v] = std::min(best[v], ms);
       }
     }
@@ -251,7 +251,7 @@ int batch_autotune(dg3d_batch* b) {
       if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
       if (e != hipSuccess) return b->hip_fail(e, "batch autotune");
       float ms = 0.f;
-      hipEventElapsedTime(&ms, b->ev0, b->ev1);
+      (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
       sbest[sc] = std::min(sbest[sc], ms);
     }
   }
@@ -326,9 +326,13 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
     return bail(DG3D_E_NOMEM, "batch: state arena allocation failed");
   const size_t du_bytes = (precision == DG3D_MIXED ? 4 : 8) * (size_t)slot;
   if (hipMalloc(&b->du, du_bytes) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: du arena allocation failed");
-  hipMemset(b->arena[0], 0, sizeof(double) * slot);
-  hipMemset(b->arena[1], 0, sizeof(double) * slot);
-  hipMemset(b->du, 0, du_bytes);
+  hipError_t up = hipSuccess;  // first failure of the uploads below, reported after them
+  auto upload = [&up](hipError_t r) {
+    if (up == hipSuccess) up = r;
+  };
+  upload(hipMemset(b->arena[0], 0, sizeof(double) * slot));
+  upload(hipMemset(b->arena[1], 0, sizeof(double) * slot));
+  upload(hipMemset(b->du, 0, du_bytes));
   // operators per order
   for (int i = 0; i < count; ++i) {
     const int n = b->solves[i].n;
@@ -343,8 +347,8 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
       std::vector<float> f(srcs[k]->begin(), srcs[k]->end());
       if (bmalloc(&oo.opd[k], cnt) != hipSuccess || bmalloc(&oo.opf[k], cnt) != hipSuccess)
         return bail(DG3D_E_NOMEM, "batch: operator allocation failed");
-      hipMemcpy(oo.opd[k], srcs[k]->data(), sizeof(double) * cnt, hipMemcpyHostToDevice);
-      hipMemcpy(oo.opf[k], f.data(), sizeof(float) * cnt, hipMemcpyHostToDevice);
+      upload(hipMemcpy(oo.opd[k], srcs[k]->data(), sizeof(double) * cnt, hipMemcpyHostToDevice));
+      upload(hipMemcpy(oo.opf[k], f.data(), sizeof(float) * cnt, hipMemcpyHostToDevice));
     }
   }
   // per-solve device data and launch records
@@ -367,7 +371,7 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
       double* p = nullptr;
       if (bmalloc(&p, iw.size()) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: geometry allocation failed");
       b->allocs.push_back(p);
-      hipMemcpy(p, iw.data(), sizeof(double) * iw.size(), hipMemcpyHostToDevice);
+      upload(hipMemcpy(p, iw.data(), sizeof(double) * iw.size(), hipMemcpyHostToDevice));
       m.inv_w[a] = p;
     }
     m.E = h.E;
@@ -392,7 +396,7 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
     double* mat = nullptr;
     if (bmalloc(&mat, rec.size()) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: material allocation failed");
     b->allocs.push_back(mat);
-    hipMemcpy(mat, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice);
+    upload(hipMemcpy(mat, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice));
     m.mat = mat;
     const double* traces[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     for (int f = 0; f < 6; ++f) {
@@ -403,7 +407,7 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
       double* p = nullptr;
       if (bmalloc(&p, cnt) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: halo allocation failed");
       b->allocs.push_back(p);
-      hipMemcpy(p, d.halo[f], sizeof(double) * cnt, hipMemcpyHostToDevice);
+      upload(hipMemcpy(p, d.halo[f], sizeof(double) * cnt, hipMemcpyHostToDevice));
       traces[f] = p;
       h.has_halo[f] = true;
     }
@@ -440,7 +444,7 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
     if (e != hipSuccess) return bail(DG3D_E_HIP, std::string("batch: state upload: ") + hipGetErrorString(e));
   }
   if (bmalloc(&b->d_records, recs.size()) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: record allocation failed");
-  hipMemcpy(b->d_records, recs.data(), sizeof(dg3d::RhsLaunch) * recs.size(), hipMemcpyHostToDevice);
+  upload(hipMemcpy(b->d_records, recs.data(), sizeof(dg3d::RhsLaunch) * recs.size(), hipMemcpyHostToDevice));
   // one group per order present, largest first; task lists for both shapes
   std::vector<int> orders;
   for (const HostSolve& h : b->solves)
@@ -463,7 +467,7 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
         b->groups.push_back(g);
         return bail(DG3D_E_NOMEM, "batch: task list allocation failed");
       }
-      hipMemcpy(g.d_tasks[v], tasks.data(), sizeof(dg3d::BatchTask) * tasks.size(), hipMemcpyHostToDevice);
+      upload(hipMemcpy(g.d_tasks[v], tasks.data(), sizeof(dg3d::BatchTask) * tasks.size(), hipMemcpyHostToDevice));
       g.ntasks[v] = (int)tasks.size();
     }
     for (int v = 0; v < dg3d::kNumBatchVariants; ++v) {
@@ -478,6 +482,7 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
   }
   set_schedule(b, kScheduleBranches);
   e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = up;
   if (e != hipSuccess) return bail(DG3D_E_HIP, std::string("batch setup: ") + hipGetErrorString(e));
   *out = b;
   return DG3D_OK;
@@ -551,7 +556,7 @@ int dg3d_batch_step(dg3d_batch* b, int nsteps) {
   if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
   if (e != hipSuccess) return b->hip_fail(e, "batch step");
   float ms = 0.f;
-  hipEventElapsedTime(&ms, b->ev0, b->ev1);
+  (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
   b->last_ms = ms;
   b->last_launches = 5 * nsteps;
   return DG3D_OK;

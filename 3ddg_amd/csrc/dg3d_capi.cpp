@@ -166,7 +166,7 @@ hipError_t dmalloc(T** p, size_t count) {
 
 void drop_graphs(dg3d_ctx* c) {
   for (StepGraph& g : c->graphs) {
-    if (g.exec) hipGraphExecDestroy(g.exec);
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
     g = StepGraph{};
   }
 }
@@ -175,31 +175,31 @@ void free_ctx(dg3d_ctx* c) {
   if (!c) return;
   drop_graphs(c);
   for (auto& r : c->regs) {
-    hipFree(r.d);
-    for (auto* h : r.halo) hipFree(h);
+    (void)hipFree(r.d);
+    for (auto* h : r.halo) (void)hipFree(h);
   }
-  hipFree(c->work_q);
-  hipFree(c->work_du);
-  hipFree(c->work_k1);
-  hipFree(c->work_k2);
+  (void)hipFree(c->work_q);
+  (void)hipFree(c->work_du);
+  (void)hipFree(c->work_k1);
+  (void)hipFree(c->work_k2);
   for (int a = 0; a < 3; ++a) {
-    hipFree(c->d_inv_w[a]);
-    hipFree(c->d_opd[a]);
-    hipFree(c->d_opf[a]);
+    (void)hipFree(c->d_inv_w[a]);
+    (void)hipFree(c->d_opd[a]);
+    (void)hipFree(c->d_opf[a]);
   }
-  hipFree(c->d_mat);
-  hipFree(c->d_diff);
-  hipFree(c->d_wq);
-  hipFree(c->d_lead);
-  hipFree(c->d_src_trace);
-  hipFree(c->d_probe_nodes);
-  hipFree(c->d_probe_buf);
-  hipFree(c->d_clock);
-  hipFree(c->d_probe_step);
-  hipFree(c->d_part);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->stream) hipStreamDestroy(c->stream);
+  (void)hipFree(c->d_mat);
+  (void)hipFree(c->d_diff);
+  (void)hipFree(c->d_wq);
+  (void)hipFree(c->d_lead);
+  (void)hipFree(c->d_src_trace);
+  (void)hipFree(c->d_probe_nodes);
+  (void)hipFree(c->d_probe_buf);
+  (void)hipFree(c->d_clock);
+  (void)hipFree(c->d_probe_step);
+  (void)hipFree(c->d_part);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -355,11 +355,11 @@ int step_graph(dg3d_ctx* c, int scheme, double dt, int steps, double* cur, doubl
   for (int i = 0; i < steps && !rc; ++i) rc = launch_step(c, scheme, dt, &a, &b, false);
   e = hipStreamEndCapture(c->stream, &graph);
   if (rc || e != hipSuccess) {
-    if (graph) hipGraphDestroy(graph);
+    if (graph) (void)hipGraphDestroy(graph);
     return rc ? rc : c->hip_fail(e, "step graph capture");
   }
   e = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
-  hipGraphDestroy(graph);
+  (void)hipGraphDestroy(graph);
   if (e != hipSuccess) {
     slot->exec = nullptr;
     return c->hip_fail(e, "step graph instantiate");
@@ -440,7 +440,7 @@ int autotune(dg3d_ctx* c, int scheme) {
         if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
         if (e != hipSuccess) return c->hip_fail(e, "autotune");
         float ms = 0.f;
-        hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
         float& b = best[v * dg3d::kNumOrders + x];
         b = std::min(b, ms);
       }
@@ -553,6 +553,10 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
       free_ctx(c);
       return DG3D_E_HIP;
     }
+    hipError_t up = hipSuccess;  // first failure of the uploads below, reported after them
+    auto upload = [&up](hipError_t r) {
+      if (up == hipSuccess) up = r;
+    };
     const std::vector<double>* srcs[3] = {&op.A, &op.L0, &op.LN};
     for (int i = 0; i < 3; ++i) {
       const size_t cnt = srcs[i]->size();
@@ -562,14 +566,14 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
         free_ctx(c);
         return DG3D_E_NOMEM;
       }
-      hipMemcpy(c->d_opd[i], srcs[i]->data(), sizeof(double) * cnt, hipMemcpyHostToDevice);
-      hipMemcpy(c->d_opf[i], f.data(), sizeof(float) * cnt, hipMemcpyHostToDevice);
+      upload(hipMemcpy(c->d_opd[i], srcs[i]->data(), sizeof(double) * cnt, hipMemcpyHostToDevice));
+      upload(hipMemcpy(c->d_opf[i], f.data(), sizeof(float) * cnt, hipMemcpyHostToDevice));
     }
     for (int a = 0; a < 3; ++a) {
       c->h_w[a].assign(c->nE[a], 1.0);
       std::vector<double> iw(c->nE[a], 1.0);
       if (dmalloc(&c->d_inv_w[a], c->nE[a]) != hipSuccess) { g_create_error = "geometry allocation failed"; free_ctx(c); return DG3D_E_NOMEM; }
-      hipMemcpy(c->d_inv_w[a], iw.data(), sizeof(double) * iw.size(), hipMemcpyHostToDevice);
+      upload(hipMemcpy(c->d_inv_w[a], iw.data(), sizeof(double) * iw.size(), hipMemcpyHostToDevice));
     }
     c->h_mat.assign((size_t)c->E * 4, 0.0);
     for (int64_t i = 0; i < c->E; ++i) {
@@ -584,9 +588,9 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
       free_ctx(c);
       return DG3D_E_NOMEM;
     }
-    hipMemcpy(c->d_wq, c->weights.data(), sizeof(double) * c->n, hipMemcpyHostToDevice);
-    hipMemset(c->d_clock, 0, sizeof(double));
-    hipMemset(c->d_probe_step, 0, sizeof(int));
+    upload(hipMemcpy(c->d_wq, c->weights.data(), sizeof(double) * c->n, hipMemcpyHostToDevice));
+    upload(hipMemset(c->d_clock, 0, sizeof(double)));
+    upload(hipMemset(c->d_probe_step, 0, sizeof(int)));
     if (upload_material(c) != DG3D_OK) { g_create_error = c->err; free_ctx(c); return DG3D_E_HIP; }
     c->regs.resize(d->num_registers);
     for (auto& r : c->regs) {
@@ -595,10 +599,11 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
         free_ctx(c);
         return DG3D_E_NOMEM;
       }
-      hipMemsetAsync(r.d, 0, sizeof(double) * c->nfields * c->fs, c->stream);
+      upload(hipMemsetAsync(r.d, 0, sizeof(double) * c->nfields * c->fs, c->stream));
     }
     c->src.face = -1;
     e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = up;
     if (e != hipSuccess) { g_create_error = std::string("HIP: ") + hipGetErrorString(e); free_ctx(c); return DG3D_E_HIP; }
   } catch (const std::exception& ex) {
     g_create_error = ex.what();
@@ -610,7 +615,7 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
 }
 
 void dg3d_destroy(dg3d_ctx* ctx) {
-  if (ctx && ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_ctx(ctx);
 }
 
@@ -784,10 +789,11 @@ int dg3d_halo(dg3d_ctx* c, int reg) {
 
 int dg3d_set_source(dg3d_ctx* c, const dg3d_source* s) {
   if (!c) return DG3D_E_INVALID;
-  hipStreamSynchronize(c->stream);
+  const hipError_t se = hipStreamSynchronize(c->stream);  // launches in flight may read the old trace
+  if (se != hipSuccess) return c->hip_fail(se, "set_source");
   drop_graphs(c);
-  hipFree(c->d_lead);
-  hipFree(c->d_src_trace);
+  (void)hipFree(c->d_lead);
+  (void)hipFree(c->d_src_trace);
   c->d_lead = nullptr;
   c->d_src_trace = nullptr;
   if (!s || s->face < 0) {
@@ -826,8 +832,9 @@ int dg3d_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t) {
   if (scheme == DG3D_LSRK54) {
     // the new state becomes register 0; the old buffer becomes the work register
     if (c->nfields > 4) {
-      hipMemcpyAsync(cur + 4 * c->fs, c->regs[0].d + 4 * c->fs, sizeof(double) * 3 * c->fs, hipMemcpyDeviceToDevice,
-                     c->stream);
+      const hipError_t e = hipMemcpyAsync(cur + 4 * c->fs, c->regs[0].d + 4 * c->fs, sizeof(double) * 3 * c->fs,
+                                          hipMemcpyDeviceToDevice, c->stream);
+      if (e != hipSuccess) return c->hip_fail(e, "stage");
     }
     c->work_q = c->regs[0].d;
     c->regs[0].d = cur;
@@ -855,9 +862,8 @@ int dg3d_step(dg3d_ctx* c, int scheme, double dt, int nsteps, double* t) {
   double* cur = c->regs[0].d;
   double* other = c->work_q;
   double tt = *t;
-  hipEventRecord(c->ev0, c->stream);
-  hipError_t e = hipSuccess;
-  if (c->src.face >= 0) e = dg3d::launch_clock_set(c->d_clock, tt, c->stream);
+  hipError_t e = hipEventRecord(c->ev0, c->stream);
+  if (e == hipSuccess && c->src.face >= 0) e = dg3d::launch_clock_set(c->d_clock, tt, c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "step");
   bool halos = false;
   for (bool v : c->regs[0].halo_valid) halos = halos || v;
@@ -886,18 +892,18 @@ int dg3d_step(dg3d_ctx* c, int scheme, double dt, int nsteps, double* t) {
     }
     s += run;
   }
-  hipEventRecord(c->ev1, c->stream);
-  if (scheme == DG3D_LSRK54 && cur != c->regs[0].d) {
-    if (c->nfields > 4)
-      hipMemcpyAsync(cur + 4 * c->fs, c->regs[0].d + 4 * c->fs, sizeof(double) * 3 * c->fs, hipMemcpyDeviceToDevice,
-                     c->stream);
+  e = hipEventRecord(c->ev1, c->stream);
+  if (e == hipSuccess && scheme == DG3D_LSRK54 && cur != c->regs[0].d) {
+    if (c->nfields > 4)  // Q rides along in register 0 (Element.cpp:288-307)
+      e = hipMemcpyAsync(cur + 4 * c->fs, c->regs[0].d + 4 * c->fs, sizeof(double) * 3 * c->fs,
+                         hipMemcpyDeviceToDevice, c->stream);
     c->work_q = c->regs[0].d;
     c->regs[0].d = cur;
   }
-  e = hipEventSynchronize(c->ev1);
+  if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
   if (e != hipSuccess) return c->hip_fail(e, "step");
   float ms = 0.f;
-  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
   c->last_ms = ms;
   c->last_launches = nsteps * ns;
   *t = tt;
@@ -985,12 +991,13 @@ int dg3d_reduce(dg3d_ctx* c, int kind, int reg, int reg2, double* out) {
 
 int dg3d_set_probes(dg3d_ctx* c, int count, const int64_t* nodes, int capacity_steps) {
   if (!c || count < 0 || capacity_steps < 0) return DG3D_E_INVALID;
-  hipStreamSynchronize(c->stream);
+  const hipError_t se = hipStreamSynchronize(c->stream);  // launches in flight may write the old buffer
+  if (se != hipSuccess) return c->hip_fail(se, "set_probes");
   drop_graphs(c);
   hipError_t e0 = hipMemsetAsync(c->d_probe_step, 0, sizeof(int), c->stream);
   if (e0 != hipSuccess) return c->hip_fail(e0, "set_probes");
-  hipFree(c->d_probe_nodes);
-  hipFree(c->d_probe_buf);
+  (void)hipFree(c->d_probe_nodes);
+  (void)hipFree(c->d_probe_buf);
   c->d_probe_nodes = nullptr;
   c->d_probe_buf = nullptr;
   c->probe_count = 0;
@@ -1005,7 +1012,8 @@ int dg3d_set_probes(dg3d_ctx* c, int count, const int64_t* nodes, int capacity_s
   if (dmalloc(&c->d_probe_nodes, count) != hipSuccess ||
       dmalloc(&c->d_probe_buf, (size_t)count * std::max(capacity_steps, 1)) != hipSuccess)
     return c->fail(DG3D_E_NOMEM, "probe allocation failed");
-  hipMemcpy(c->d_probe_nodes, slots.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice);
+  const hipError_t pe = hipMemcpy(c->d_probe_nodes, slots.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice);
+  if (pe != hipSuccess) return c->hip_fail(pe, "set_probes");
   c->probe_count = count;
   c->probe_cap = capacity_steps;
   return DG3D_OK;
