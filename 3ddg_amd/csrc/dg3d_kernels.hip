@@ -534,7 +534,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
         const bool state = inner || tr == nullptr;
         item[r] |= (!inner && tr == nullptr) ? kItemExterior : 0u;
         // a neighbour in this workgroup: phase 2 reads its kernel functions from LDS, no load
-        const bool local = !LDG && inner && nb >= e0 && nb < e0 + EPB;
+        const bool local = inner && nb >= e0 && nb < e0 + EPB;
         item[r] |= local ? (kItemLocal | ((unsigned)(nb - e0) << 25)) : 0u;
         if (!local) {
           const int64_t oi = inner ? (int64_t)nb * ES + on : (int64_t)he * ES + sn;
@@ -621,7 +621,13 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     if constexpr (LDG) {
       double vq = nq[r];
       if constexpr (!std::is_same<CT, double>::value) asm volatile("" : "+v"(vq));
-      const CT qs = sm.sQ[(fle * 3 + axis) * NP + sn], qo = zero ? CT(0) : (CT)vq;
+      CT qo = zero ? CT(0) : (CT)vq;
+      if (it & kItemLocal) {
+        const int b = fn / NN, cc = fn - (fn / NN) * NN;
+        const int on = face_node<NN>(axis, (dir & 1) ? 0 : NN - 1, b, cc);
+        qo = sm.sQ[(((it >> 25) & 31u) * 3 + axis) * NP + on];
+      }
+      const CT qs = sm.sQ[(fle * 3 + axis) * NP + sn];
       sm.sFq[(fle * 6 + dir) * NF2 + fn] = (dir & 1) ? CT(-0.5) * (qs + qo) : CT(0.5) * (qo + qs);
     }
   }
@@ -881,50 +887,108 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
 // (rho'_o + rho'_s)/2 and F_next = -(rho'_s + rho'_o)/2 (the exterior trace on block faces,
 // the source included). FP64, one lane per node; writes fields 4..6, reads field 0.
 template <int NN>
-__global__ void __launch_bounds__(Shape<NN>::NT) k_ldg_q(const LdgLaunch L) {
-  using S = Shape<NN>;
-  constexpr int NP = S::NP, NF2 = S::NF2, EPB = S::EPB, ES = S::ES;
+__global__ void __launch_bounds__((Geo<NN, Shape<NN>::EPB>::NT)) k_ldg_q(const LdgLaunch L) {
+  constexpr int EPB = Shape<NN>::EPB;
+  using S = Geo<NN, EPB>;
+  constexpr int NP = S::NP, NF2 = S::NF2, ES = S::ES, NT = S::NT;
+  constexpr int NITEMS = EPB * 6 * NF2;
+  constexpr int R = (NITEMS + NT - 1) / NT;
   __shared__ double sV[EPB * NP];
   __shared__ double sFl[EPB * 6 * NF2];
+  __shared__ double sA[NN * NN], sL0[NN], sLN[NN], sIw[EPB * 3];
   const MeshArgs& m = L.mesh;
+  const int E = (int)m.E;
   const int tid = threadIdx.x;
   const int e0 = (int)blockIdx.x * EPB;
-  const int le = tid / NP;
-  const int node = tid - le * NP;
+  const int le = tid / ES;
+  const int node = tid - le * ES;
   const int e = e0 + le;
-  const bool valid = (le < EPB) && (e < (int)m.E);
-  const int64_t gi = (int64_t)e * ES + node;
-  if (valid) sV[le * NP + node] = L.in[gi];
-  __syncthreads();
-  for (int f = tid; f < EPB * 6 * NF2; f += S::NT) {
-    const int fle = f / (6 * NF2);
-    const int r = f - fle * 6 * NF2;
-    const int dir = r / NF2;
-    const int fn = r - dir * NF2;
-    const int fe = e0 + fle;
-    if (fe >= (int)m.E) continue;
-    const int axis = 2 - (dir >> 1), side = dir & 1;
-    const int b = fn / NN, cc = fn - (fn / NN) * NN;
-    const int sn = face_node<NN>(axis, side ? NN - 1 : 0, b, cc);
-    const int on = face_node<NN>(axis, side ? 0 : NN - 1, b, cc);
-    const double vs = sV[fle * NP + sn];
-    int ec[3];
-    elem_coords(m, (uint32_t)fe, ec);
-    const int nb = face_neighbour(m, fe, ec, dir);
-    double vo = 0.0;
-    if (nb >= 0) {
-      vo = L.in[(int64_t)nb * ES + on];
-    } else {
-      const double* tr = L.ext.trace[dir];
-      if (tr) vo = tr[((int64_t)(-1 - nb) * NF2 + fn) * 3];
+  const bool valid = (le < EPB) && (e < E) && node < NP;
+  const int64_t gi = (int64_t)(valid ? e : 0) * ES + node;
+
+  // ---- phase 0: every global load of the workgroup (as in the stage kernel) ---------------
+  const double v = valid ? L.in[gi] : 0.0;
+  const double* trace_ptr[6];
+#pragma unroll
+  for (int d = 0; d < 6; ++d) trace_ptr[d] = L.ext.trace[d];
+  double vo[R];
+  unsigned item[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int f = tid + r * NT;
+    vo[r] = 0.0;
+    item[r] = ~0u;
+    if (f < NITEMS) {
+      const int fle = f / (6 * NF2);
+      const int rr = f - fle * (6 * NF2);
+      const int dir = rr / NF2;
+      const int fn = rr - dir * NF2;
+      const int fe = e0 + fle;
+      if (fe < E) {
+        const int axis = 2 - (dir >> 1);
+        const int b = fn / NN, cc = fn - (fn / NN) * NN;
+        const int sn = face_node<NN>(axis, (dir & 1) ? NN - 1 : 0, b, cc);
+        const int on = face_node<NN>(axis, (dir & 1) ? 0 : NN - 1, b, cc);
+        int ec[3];
+        elem_coords(m, (uint32_t)fe, ec);
+        const int nb = face_neighbour(m, fe, ec, dir);
+        const double* tr = trace_ptr[0];
+#pragma unroll
+        for (int d = 1; d < 6; ++d)
+          if (dir == d) tr = trace_ptr[d];
+        // the neighbour's rho', or the exterior trace's; an exterior face without a trace reads
+        // the lane's own node and is zeroed below (kItemExterior)
+        const bool inner = nb >= 0;
+        const bool state = inner || tr == nullptr;
+        const int64_t oi = inner ? (int64_t)nb * ES + on : (int64_t)fe * ES + sn;
+        const double* pv = state ? L.in + oi : tr + ((int64_t)(-1 - nb) * NF2 + fn) * 3;
+        vo[r] = *pv;
+        item[r] = (unsigned)fle | ((unsigned)dir << 6) | ((unsigned)fn << 9) | ((unsigned)sn << 16) |
+                  ((!inner && tr == nullptr) ? kItemExterior : 0u);
+      }
     }
-    sFl[(fle * 6 + dir) * NF2 + fn] = side ? -0.5 * (vs + vo) : 0.5 * (vo + vs);
+  }
+  double av = 0.0, l0v = 0.0, lnv = 0.0, iwv = 0.0;
+  if (tid < NN * NN) av = L.opd[0][tid];
+  if (tid < NN) {
+    l0v = L.opd[1][tid];
+    lnv = L.opd[2][tid];
+  }
+  if (tid < EPB * 3) {
+    const int he = e0 + tid / 3, a = tid - (tid / 3) * 3;
+    if (he < E) {
+      int ec[3];
+      elem_coords(m, (uint32_t)he, ec);
+      iwv = sel3(a, m.inv_w[0], m.inv_w[1], m.inv_w[2])[sel3(a, ec[0], ec[1], ec[2])];
+    }
+  }
+
+  // ---- phase 1: the workgroup's rho', operators and 1/width into LDS ----------------------
+  if (valid) sV[le * NP + node] = v;
+  if (tid < NN * NN) sA[tid] = av;
+  if (tid < NN) {
+    sL0[tid] = l0v;
+    sLN[tid] = lnv;
+  }
+  if (tid < EPB * 3) sIw[tid] = iwv;
+  __syncthreads();
+
+  // ---- phase 2: central fluxes of the face nodes -----------------------------------------
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const unsigned it = item[r];
+    if (it == ~0u) continue;
+    const int fle = it & 63u, dir = (it >> 6) & 7u, fn = (it >> 9) & 127u;
+    const int sn = (it >> 16) & 0x1FFu;
+    const double vs = sV[fle * NP + sn];
+    const double o = (it & kItemExterior) ? 0.0 : vo[r];
+    sFl[(fle * 6 + dir) * NF2 + fn] = (dir & 1) ? -0.5 * (vs + o) : 0.5 * (o + vs);
   }
   __syncthreads();
   if (!valid) return;
+
+  // ---- phase 3: Q_a = -(1/w_a) (A rho' + L0 F_prev + LN F_next) ---------------------------
   const int idx[3] = {node / NF2, (node / NN) % NN, node % NN};
-  int ec[3];
-  elem_coords(m, (uint32_t)e, ec);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int ix = idx[a];
@@ -933,9 +997,10 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg_q(const LdgLaunch L) {
     const int fn = (a == 0) ? idx[1] * NN + idx[2] : (a == 1 ? idx[0] * NN + idx[2] : idx[0] * NN + idx[1]);
     const int dlo = (a == 0) ? 4 : (a == 1 ? 2 : 0);
     double sv = 0.0;
-    for (int mm = 0; mm < NN; ++mm) sv += L.opd[0][ix * NN + mm] * sV[le * NP + base + mm * ls];
-    sv += sFl[(le * 6 + dlo) * NF2 + fn] * L.opd[1][ix] + sFl[(le * 6 + dlo + 1) * NF2 + fn] * L.opd[2][ix];
-    L.out[(4 + a) * m.fs + gi] = -(sv * m.inv_w[a][ec[a]]);
+#pragma unroll
+    for (int mm = 0; mm < NN; ++mm) sv += sA[ix * NN + mm] * sV[le * NP + base + mm * ls];
+    sv += sFl[(le * 6 + dlo) * NF2 + fn] * sL0[ix] + sFl[(le * 6 + dlo + 1) * NF2 + fn] * sLN[ix];
+    L.out[(4 + a) * m.fs + gi] = -(sv * sIw[le * 3 + a]);
   }
 }
 
@@ -1065,10 +1130,9 @@ __global__ void k_step_end(double* clock, double dt, int* probe_step, int cap) {
 }
 #endif  // DG3D_TU_ORDER
 
-// The fused stage with the LDG diffusion term: default shape, VALU contraction.
-template <int NN>
-hipError_t launch_rhs_ldg(const RhsLaunch& L, hipStream_t s) {
-  constexpr int EPB = Variants<NN>::EPB0;
+// The fused stage with the LDG diffusion term, VALU contraction.
+template <int NN, int EPB>
+hipError_t launch_rhs_ldg_e(const RhsLaunch& L, hipStream_t s) {
   using S = Geo<NN, EPB>;
   const int64_t blocks = (L.mesh.E + EPB - 1) / EPB;
   if (blocks <= 0) return hipSuccess;
@@ -1084,6 +1148,14 @@ hipError_t launch_rhs_ldg(const RhsLaunch& L, hipStream_t s) {
     return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// The stage with the LDG term: the default elements-per-workgroup shape, VALU contraction, no
+// occupancy target, early RK-register loads. (The alternative shape, which the tuner often picks
+// for the stage without diffusion, measured 2 % slower with it at 64^3 N=4.)
+template <int NN>
+hipError_t launch_rhs_ldg(const RhsLaunch& L, hipStream_t s) {
+  return launch_rhs_ldg_e<NN, Variants<NN>::EPB0>(L, s);
 }
 
 template <int NN, int EPB, int MINW, bool LATE>
@@ -1133,10 +1205,10 @@ hipError_t launch_rhs_n(const RhsLaunch& L, hipStream_t s) {
 
 template <int NN>
 hipError_t launch_ldg_q_n(const LdgLaunch& L, hipStream_t s) {
-  using S = Shape<NN>;
-  const int64_t blocks = (L.mesh.E + S::EPB - 1) / S::EPB;
+  constexpr int EPB = Shape<NN>::EPB;
+  const int64_t blocks = (L.mesh.E + EPB - 1) / EPB;
   if (blocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_ldg_q<NN>), dim3((unsigned)blocks), dim3(S::NT), 0, s, L);
+  hipLaunchKernelGGL((k_ldg_q<NN>), dim3((unsigned)blocks), dim3(Geo<NN, EPB>::NT), 0, s, L);
   return hipGetLastError();
 }
 

@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-solve workload")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched 256-solve workload")
     ap.add_argument("--case", choices=["c3m_valu", "c3m_mfma_f32", "c3m_mfma_bf16", "c5m_valu", "c5m_mfma_f32",
-                                       "c5m_mfma_bf16", "c5_fp64"], default=None,
+                                       "c5m_mfma_bf16", "c5_fp64", "c2_rk33", "c2_ldg"], default=None,
                     help="time one workload alone (for profiling): 64^3 N=7 or 48^3 N=5, mixed with the VALU or a "
                          "matrix-core contraction, or 48^3 N=5 FP64; prints one JSON line")
     ap.add_argument("--c5-steps", type=int, default=2000, help="steps of the combined 48^3 N=5 workload (0: skip)")
@@ -73,28 +73,54 @@ def calibrate_copy():
                       "copy_GBs": 2 * n * 8 / el / 1e9}))
 
 
-def timed_run(pk, p, precision, steps, warmup, probe=None, variant=-1, xcd=1, contraction=0):
-    ctx = pk.problem.make_context(p, precision=precision, contraction=contraction)
+def timed_run(pk, p, precision, steps, warmup, probe=None, variant=-1, xcd=1, contraction=0, scheme=None,
+              with_q=False):
+    scheme = pk.dg3d.LSRK54 if scheme is None else scheme
+    ctx = pk.problem.make_context(p, precision=precision, contraction=contraction, with_q=with_q)
     dt = ctx.cfl_dt(p.cfl)
     if variant >= 0:
-        ctx.set_variant(pk.dg3d.LSRK54, variant, xcd)
+        ctx.set_variant(scheme, variant, xcd)
     else:
-        variant, xcd = ctx.autotune(pk.dg3d.LSRK54)
+        variant, xcd = ctx.autotune(scheme)
     if probe is not None:
         ctx.set_probes([probe], steps + warmup)
     t = 0.0
     if warmup:
-        t = ctx.step(pk.dg3d.LSRK54, dt, warmup, t)
+        t = ctx.step(scheme, dt, warmup, t)
     ctx.sync()
     import torch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    t = ctx.step(pk.dg3d.LSRK54, dt, steps, t)
+    t = ctx.step(scheme, dt, steps, t)
     ctx.sync()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     stage_ms, launches = ctx.last_step_timing()
     return ctx, dict(elapsed=el, stage_ms=stage_ms, launches=launches, dt=dt, t_end=t, variant=variant, xcd=xcd)
+
+
+def other_paths_run(pk, elements, steps, warmup):
+    """The 64^3 N=4 FP64 workload through the reference's own scheme (RK33, Grid.cpp:642-835:
+    64, 96 and 96 B per node for its three stages, 85.3 B on average) and through LSRK54 with LDG
+    diffusion on (per stage the Q kernel, 32 B per node, and the stage, which also reads Q:
+    121.6 + 24 B; 177.6 B per node-stage in all). Rates per stage launch group."""
+    out = {}
+    p = pk.problem.config_c2(num_elements=elements, steps=steps)
+    ctx, r = timed_run(pk, p, pk.dg3d.FP64, steps, warmup, scheme=pk.dg3d.RK33)
+    ctx.close()
+    bpn = (64.0 + 96.0 + 96.0) / 3.0
+    out["rk33_fp64"] = dict(stage_ms=r["stage_ms"], variant=[r["variant"], r["xcd"]], bytes_per_node_stage=bpn,
+                            dof_stage_per_s=p.dof * 3 * steps / r["elapsed"],
+                            hbm_frac=p.dof * bpn / (r["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+    p = pk.problem.config_c2(num_elements=elements, steps=steps)
+    p.diffusion = np.full(p.E, 1.0e-6)
+    ctx, r = timed_run(pk, p, pk.dg3d.FP64, steps, warmup, with_q=True)
+    ctx.close()
+    bpn = 32.0 + BYTES_FP64 + 24.0
+    out["lsrk54_ldg_fp64"] = dict(stage_ms=r["stage_ms"], variant=[r["variant"], r["xcd"]], bytes_per_node_stage=bpn,
+                                  dof_stage_per_s=p.dof * 5 * steps / r["elapsed"],
+                                  hbm_frac=p.dof * bpn / (r["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+    return out
 
 
 def small_block_run(pk, steps=2000, warmup=20):
@@ -290,6 +316,16 @@ def main():
     if args.batch_only:
         print(json.dumps({"batch_c4": batch_run(pk, args.batch_steps)}))
         return
+    if args.case in ("c2_rk33", "c2_ldg"):
+        p = pk.problem.config_c2(num_elements=args.elements, steps=args.steps)
+        if args.case == "c2_ldg":
+            p.diffusion = np.full(p.E, 1.0e-6)
+        ctx, r = timed_run(pk, p, pk.dg3d.FP64, args.steps, args.warmup, variant=args.variant, xcd=args.xcd,
+                           scheme=pk.dg3d.RK33 if args.case == "c2_rk33" else None, with_q=args.case == "c2_ldg")
+        ctx.close()
+        print(json.dumps(dict(case=args.case, dof=p.dof, steps=args.steps, stage_ms=r["stage_ms"],
+                              variant=[r["variant"], r["xcd"]])))
+        return
     if args.case:
         name, rest = args.case.split("_", 1)
         p = (pk.problem.config_c3 if name == "c3m" else pk.problem.config_c5)(steps=args.steps)
@@ -349,6 +385,7 @@ def main():
                                              args.steps, args.warmup)
     if not args.no_extra:
         out["small_c1"] = small_block_run(pk)
+        out["other_paths_c2"] = other_paths_run(pk, args.elements, min(args.steps, 200), args.warmup)
     if not args.no_batch and not args.no_extra:
         out["batch_c4"] = batch_run(pk, args.batch_steps)
     if not args.no_extra and args.c5_steps > 0:
