@@ -2,11 +2,17 @@
 // loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the
 // product. Each function cites the reference lines it restates.
 //
-// Parity pins (tests/test_oracle_*.py): LGL data and M, S against numpy.polynomial and
-// closed forms; the per-line solve against the reference's own Matrix.h QR solve
-// (oracle/_ref); line operator polynomial exactness and the upwind interface
-// transmission/reflection closed form; sponge sigma tables by brute force; SSP-RK3 / LSRK54
-// orders on y'=ly; energy non-increase; literal vs fp64 agreement at FP32 level.
+// Parity pins (tests/test_oracle_*.py, tests/test_basis.py):
+//   LGL data and M, S: numpy.polynomial and closed forms;
+//   per-line solve: the reference's own Matrix.h QR solve (oracle/_ref);
+//   line operator: polynomial exactness; upwind flux: the interface transmission/reflection
+//     closed form; energy non-increase;
+//   sponge sigma: tables by brute force; SSP-RK3 / LSRK54: orders 3 and 4 on y' = -sigma y;
+//   LDG (QFromDRho / DRhoFromQ): exactness for a cubic, heat-kernel decay of a Fourier mode;
+//   forwardEuler: order 1, one step (1 - sigma dt) exactly, equal to RK33's first stage, LDG
+//     on the x axis only;
+//   literal (FP32) mode: agreement with FP64 at FP32 level. Its bitwise FP32 rounding pattern
+//     is parity unpinned: the full reference needs MPI and HDF5 and is not built here.
 #include "oracle.h"
 
 #include <algorithm>
