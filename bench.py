@@ -288,6 +288,28 @@ def cpu_baseline(p, steps):
                        f"with per-line dense solves, {el:.1f} s")
 
 
+def cpu_literal_baseline(p, steps):
+    """The reference's own arithmetic on the CPU: the oracle in literal mode (FP32 state and
+    flux constants, per-line dense solves, Grid::RK33 stages) on `steps` RK33 steps of the same
+    workload, OpenMP over elements on this host's cores (the reference runs one block per rank)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Oracle, load_problem_into
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = Oracle(p, literal=True, num_registers=4, threads=threads)
+    load_problem_into(o, p)
+    dt = importlib.import_module("3ddg_amd").problem.cfl_dt_host(p)
+    o.set_time(0.0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.rk33(dt, 3, 0, 1, 2)
+        o.copy(0, 3)
+    el = time.perf_counter() - t0
+    del o
+    return dict(value=p.dof * 3 * steps / el, unit="DOF-stage/s", cores=threads, kind="port",
+                sample=f"{steps} RK33 step(s) (3 stages) of the full {p.name} block, literal FP32 oracle "
+                       f"(the reference's arithmetic), {el:.1f} s")
+
+
 def traffic_from_profiles(name):
     path = os.path.join(ROOT, "profiles", f"{name}.json")
     if not os.path.exists(path):
@@ -396,6 +418,8 @@ def main():
                                                 matrix_core_runs=False)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pk.problem.config_c2(num_elements=args.elements), args.cpu_steps)
+        out["cpu_literal_rk33"] = cpu_literal_baseline(pk.problem.config_c2(num_elements=args.elements),
+                                                       max(1, args.cpu_steps // 2))
     print(json.dumps(out))
 
 
