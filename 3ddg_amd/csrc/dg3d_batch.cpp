@@ -57,6 +57,14 @@ struct dg3d_batch {
   int cur = 0;                         // arena holding the state
   bool halos_pending = true;           // the next stage is the first of the run
   dg3d::RhsLaunch* d_records = nullptr;  // [solve][kBatchRecords]
+  // boundary sources: entries for the solves that have one, the solves' clocks (time of the
+  // current step), time steps and stage offsets C_s dt
+  dg3d::BatchSource* d_sources = nullptr;
+  int nsources = 0;
+  int64_t source_items = 0;  // face nodes of the largest source face
+  double* d_clocks = nullptr;
+  double* d_dts = nullptr;
+  double* d_offsets = nullptr;
   // per order present (descending n): task lists, grids and the tuned shape
   struct Group {
     int n = 0;
@@ -113,6 +121,10 @@ void free_batch(dg3d_batch* b) {
   (void)hipFree(b->arena[1]);
   (void)hipFree(b->du);
   (void)hipFree(b->d_records);
+  (void)hipFree(b->d_sources);
+  (void)hipFree(b->d_clocks);
+  (void)hipFree(b->d_dts);
+  (void)hipFree(b->d_offsets);
   for (auto& g : b->groups) {
     for (auto* t : g.d_tasks) (void)hipFree(t);
     if (g.done) (void)hipEventDestroy(g.done);
@@ -162,7 +174,10 @@ hipError_t launch_stage(dg3d_batch* b, int stage, bool halo, int from) {
   std::vector<char> used(ng, 0);
   for (const auto& g : b->groups)
     if (g.branch >= 0) used[g.branch] = 1;
-  hipError_t e = hipEventRecord(b->fork, b->stream);
+  // the source faces' exterior traces at every solve's stage time, before the stage kernels
+  hipError_t e = dg3d::launch_batch_sources(b->d_sources, b->nsources, b->source_items, b->d_clocks, b->d_offsets,
+                                            stage, halo ? 1 : 0, b->stream);
+  if (e == hipSuccess) e = hipEventRecord(b->fork, b->stream);
   for (size_t i = 0; i < ng && e == hipSuccess; ++i)
     if (used[i]) e = hipStreamWaitEvent(b->groups[i].stream, b->fork, 0);
   for (auto& g : b->groups) {
@@ -186,6 +201,8 @@ hipError_t launch_step(dg3d_batch* b, int from, bool first_halo) {
     if (e != hipSuccess) return e;
     cur = 1 - cur;
   }
+  // the solves with a source advance their clocks by their own dt
+  if (b->nsources > 0) return dg3d::launch_batch_clocks(b->d_clocks, b->d_dts, (int)b->solves.size(), b->stream);
   return hipSuccess;
 }
 
@@ -301,6 +318,11 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
     }
     if (!d.mass || !d.stiff || !d.c0 || !d.rho0 || !d.state) return bail(DG3D_E_INVALID, tag + "missing array");
     if (!(d.dt > 0.0) || !std::isfinite(d.dt)) return bail(DG3D_E_INVALID, tag + "dt must be positive");
+    if (d.source_face >= 0) {
+      if (d.source_face > 5) return bail(DG3D_E_INVALID, tag + "source face must be 0..5");
+      if (d.periodic[2 - d.source_face / 2]) return bail(DG3D_E_INVALID, tag + "source on a periodic face");
+      if (!(d.source_tau > 0.0)) return bail(DG3D_E_INVALID, tag + "source tau must be positive");
+    }
     h.n = d.order + 1;
     h.np = h.n * h.n * h.n;
     h.es = dg3d::elem_stride(h.n);
@@ -353,6 +375,8 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
   }
   // per-solve device data and launch records
   std::vector<dg3d::RhsLaunch> recs((size_t)count * dg3d::kBatchRecords);
+  std::vector<dg3d::BatchSource> sources;
+  int64_t source_items = 0;
   std::vector<double> stage;
   for (int i = 0; i < count; ++i) {
     const dg3d_solve_desc& d = descs[i];
@@ -424,6 +448,36 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
     base.dt = h.dt;
     char* du0 = static_cast<char*>(b->du) + (precision == DG3D_MIXED ? 4 : 8) * (size_t)h.slot0;
     base.du = du0;
+    // a boundary source: its face's trace buffer, which the source kernel fills every stage
+    double* src_out = nullptr;
+    if (d.source_face >= 0) {
+      const int f = d.source_face, axis = 2 - f / 2;
+      const int64_t fe = (int64_t)h.nE[axis == 0 ? 1 : 0] * h.nE[axis == 2 ? 1 : 2];
+      const int64_t items = fe * h.n * h.n;
+      if (bmalloc(&src_out, (size_t)items * 3) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: source allocation failed");
+      b->allocs.push_back(src_out);
+      double* lead = nullptr;
+      if (d.source_lead) {
+        if (bmalloc(&lead, (size_t)items) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: source allocation failed");
+        b->allocs.push_back(lead);
+        upload(hipMemcpy(lead, d.source_lead, sizeof(double) * items, hipMemcpyHostToDevice));
+      }
+      dg3d::BatchSource bs{};
+      bs.x.src_face = f;
+      bs.x.amp = d.source_amplitude;
+      bs.x.freq = d.source_frequency;
+      bs.x.phase = d.source_phase;
+      bs.x.t0 = d.source_t0;
+      bs.x.tau = d.source_tau;
+      bs.x.lead = lead;
+      bs.m = m;
+      bs.n = h.n;
+      bs.solve = i;
+      bs.halo = traces[f];
+      bs.out = src_out;
+      sources.push_back(bs);
+      source_items = std::max(source_items, items);
+    }
     for (int r = 0; r < dg3d::kBatchRecords; ++r) {
       const int from = r & 1, st = r < 10 ? r / 2 : 0;
       dg3d::RhsLaunch L = base;
@@ -432,6 +486,7 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
       L.a = dg3d::kLsA[st];
       L.b = dg3d::kLsB[st];
       for (int f = 0; f < 6; ++f) L.ext.trace[f] = (r >= 10) ? traces[f] : nullptr;
+      if (src_out) L.ext.trace[d.source_face] = src_out;
       recs[(size_t)i * dg3d::kBatchRecords + r] = L;
     }
     // initial state: host [4][E][np] -> padded slots
@@ -442,6 +497,23 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
                     sizeof(double) * h.np);
     e = hipMemcpy(b->arena[0] + h.slot0, stage.data(), sizeof(double) * stage.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(DG3D_E_HIP, std::string("batch: state upload: ") + hipGetErrorString(e));
+  }
+  if (!sources.empty()) {
+    // clocks from 0; stage offsets C_s dt computed here, as dg3d_step computes them
+    std::vector<double> dts(count), offs((size_t)count * 5);
+    for (int i = 0; i < count; ++i) {
+      dts[i] = b->solves[i].dt;
+      for (int st = 0; st < 5; ++st) offs[(size_t)i * 5 + st] = 0.0 + dg3d::kLsC[st] * dts[i];
+    }
+    if (bmalloc(&b->d_sources, sources.size()) != hipSuccess || bmalloc(&b->d_clocks, (size_t)count) != hipSuccess ||
+        bmalloc(&b->d_dts, (size_t)count) != hipSuccess || bmalloc(&b->d_offsets, offs.size()) != hipSuccess)
+      return bail(DG3D_E_NOMEM, "batch: source allocation failed");
+    upload(hipMemcpy(b->d_sources, sources.data(), sizeof(dg3d::BatchSource) * sources.size(), hipMemcpyHostToDevice));
+    upload(hipMemset(b->d_clocks, 0, sizeof(double) * count));
+    upload(hipMemcpy(b->d_dts, dts.data(), sizeof(double) * count, hipMemcpyHostToDevice));
+    upload(hipMemcpy(b->d_offsets, offs.data(), sizeof(double) * offs.size(), hipMemcpyHostToDevice));
+    b->nsources = (int)sources.size();
+    b->source_items = source_items;
   }
   if (bmalloc(&b->d_records, recs.size()) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: record allocation failed");
   upload(hipMemcpy(b->d_records, recs.data(), sizeof(dg3d::RhsLaunch) * recs.size(), hipMemcpyHostToDevice));

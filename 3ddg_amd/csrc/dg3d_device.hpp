@@ -232,6 +232,22 @@ int batch_max_grid(int n, int variant, int mixed);  // resident workgroups on th
 hipError_t launch_source_trace(const ExtArgs& x, const MeshArgs& m, int n, const double* halo, double* out,
                                hipStream_t s);
 
+// Boundary sources of the batched path: one entry per solve with a source. A stage's launch
+// computes every entry's exterior trace at the solve's stage time clocks[solve] +
+// offsets[solve * 5 + stage] (the offsets C_s dt prepared on the host, as dg3d_step's).
+struct BatchSource {
+  ExtArgs x;           // src_face, amp .. tau, lead (t and clock unused)
+  MeshArgs m;          // nE and mat of the solve
+  int n;
+  int solve;
+  const double* halo;  // the face's first-stage halo trace, or null
+  double* out;         // the solve's trace buffer for that face
+};
+hipError_t launch_batch_sources(const BatchSource* list, int count, int64_t max_items, const double* clocks,
+                                const double* offsets, int stage, int first, hipStream_t s);
+// clocks[i] += dts[i] for every solve
+hipError_t launch_batch_clocks(double* clocks, const double* dts, int count, hipStream_t s);
+
 // LDG operators (Element.cpp:623-758) for one axis, FP64.
 struct LdgLaunch {
   int n;

@@ -785,14 +785,15 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT)) k_rhs_batch(const BatchLau
 #ifndef DG3D_TU_ORDER
 // Exterior trace of the source face at one stage time: the face's halo trace (or zero) plus the
 // boundary source, p = s(t + lead), rho' = s/c^2, u_axis = +-s/(rho0 c) entering the block.
-__global__ void k_source_trace(ExtArgs x, MeshArgs m, int n, const double* halo, double* out) {
+// Node i of the source face's trace at stage time ts.
+__device__ __forceinline__ void source_trace_node(const ExtArgs& x, const MeshArgs& m, int n, const double* halo,
+                                                  double* out, double ts, int64_t i) {
   const int64_t nf2 = (int64_t)n * n;
   const int face = x.src_face;
   const int axis = 2 - face / 2;
   const int nb = (axis == 0) ? m.nE[1] : m.nE[0];
   const int nc = (axis == 2) ? m.nE[1] : m.nE[2];
   const int64_t count = (int64_t)nb * nc * nf2;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const int64_t fidx = i / nf2;
   const int eb = (int)(fidx / nc), ec = (int)(fidx % nc);
@@ -801,7 +802,6 @@ __global__ void k_source_trace(ExtArgs x, MeshArgs m, int n, const double* halo,
   e3[axis == 0 ? 1 : 0] = eb;
   e3[axis == 2 ? 1 : 2] = ec;
   const int64_t e = ((int64_t)e3[0] * m.nE[1] + e3[1]) * m.nE[2] + e3[2];
-  const double ts = x.clock ? x.clock[0] + x.t : x.t;
   const double t = ts + (x.lead ? x.lead[i] : 0.0);
   const double sv = source_value(x, t);
   const double cd = m.mat[e * kMatStride + MAT_C], rd = m.mat[e * kMatStride + MAT_RHO0];
@@ -809,6 +809,25 @@ __global__ void k_source_trace(ExtArgs x, MeshArgs m, int n, const double* halo,
   out[i * 3 + 0] = h0 + sv / (cd * cd);
   out[i * 3 + 1] = h1 + ((face & 1) ? -1.0 : 1.0) * sv / (rd * cd);
   out[i * 3 + 2] = h2;
+}
+
+__global__ void k_source_trace(ExtArgs x, MeshArgs m, int n, const double* halo, double* out) {
+  const double ts = x.clock ? x.clock[0] + x.t : x.t;
+  source_trace_node(x, m, n, halo, out, ts, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Every batched solve's source trace for one stage: blockIdx.y picks the entry.
+__global__ void k_batch_source(const BatchSource* list, const double* clocks, const double* offsets, int stage,
+                               int first) {
+  const BatchSource& b = list[blockIdx.y];
+  const double ts = clocks[b.solve] + offsets[(int64_t)b.solve * 5 + stage];
+  source_trace_node(b.x, b.m, b.n, first ? b.halo : nullptr, b.out, ts,
+                    (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void k_batch_clocks(double* clocks, const double* dts, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) clocks[i] += dts[i];
 }
 #endif  // DG3D_TU_ORDER
 
@@ -1395,6 +1414,20 @@ hipError_t launch_source_trace(const ExtArgs& x, const MeshArgs& m, int n, const
   const int64_t nc = (axis == 2) ? m.nE[1] : m.nE[2];
   const int64_t count = nb * nc * n * n;
   hipLaunchKernelGGL(k_source_trace, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, x, m, n, halo, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_sources(const BatchSource* list, int count, int64_t max_items, const double* clocks,
+                                const double* offsets, int stage, int first, hipStream_t s) {
+  if (count <= 0 || max_items <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_source, dim3((unsigned)((max_items + 255) / 256), (unsigned)count), dim3(256), 0, s,
+                     list, clocks, offsets, stage, first);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_clocks(double* clocks, const double* dts, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_clocks, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, clocks, dts, count);
   return hipGetLastError();
 }
 

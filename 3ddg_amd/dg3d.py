@@ -67,6 +67,13 @@ class SolveDesc(ctypes.Structure):
         ("state", _dp),
         ("halo", _dp * 6),
         ("dt", ctypes.c_double),
+        ("source_face", ctypes.c_int),
+        ("source_amplitude", ctypes.c_double),
+        ("source_frequency", ctypes.c_double),
+        ("source_phase", ctypes.c_double),
+        ("source_t0", ctypes.c_double),
+        ("source_tau", ctypes.c_double),
+        ("source_lead", _dp),
     ]
 
 
@@ -373,7 +380,8 @@ class Batch:
     """Independent blocks of different sizes and orders advanced together by LSRK54 (the batched
     entry points of include/dg3d.h). `solves` is a sequence of dicts with the keys order,
     num_elements, mass, stiff, widths (3 arrays), c0, rho0, b_over_a, sigma, state (4 fields,
-    dg3d_set_state layout), dt, and optionally periodic and halo (6 traces or None)."""
+    dg3d_set_state layout), dt, and optionally periodic, halo (6 traces or None) and source (a
+    dict with face, amplitude, frequency, phase, t0, tau and optionally lead, as set_source)."""
 
     def __init__(self, solves, precision=MIXED):
         self._L = lib()
@@ -395,6 +403,15 @@ class Batch:
             for f in range(6):
                 d.halo[f] = self._arr(halo[f])
             d.dt = float(s["dt"])
+            src = s.get("source")
+            d.source_face = -1 if src is None else int(src["face"])
+            if src is not None:
+                d.source_amplitude = float(src["amplitude"])
+                d.source_frequency = float(src["frequency"])
+                d.source_phase = float(src["phase"])
+                d.source_t0 = float(src["t0"])
+                d.source_tau = float(src["tau"])
+                d.source_lead = self._arr(src.get("lead"))
             E = int(np.prod(s["num_elements"]))
             self.sizes.append((E, (order + 1) ** 3))
         h = ctypes.c_void_p()

@@ -359,12 +359,12 @@ def batch_solve(p):
                 widths=p.widths, c0=p.c0, rho0=p.rho0, b_over_a=p.b_over_a, sigma=p.sigma,
                 state=initial_state(p),
                 halo=[None if (tr is None or p.periodic[FACE_AXIS[f]]) else tr for f, tr in enumerate(halos)],
-                dt=cfl_dt_host(p))
+                dt=cfl_dt_host(p),
+                source=None if p.source.face < 0 else dict(face=p.source.face, amplitude=p.source.amplitude,
+                                                           frequency=p.source.frequency, phase=p.source.phase,
+                                                           t0=p.source.t0, tau=p.source.tau, lead=p.source.lead))
 
 
 def make_batch(problems, precision=dg3d.MIXED):
-    """A device batch of the problems (no boundary sources)."""
-    for p in problems:
-        if p.source.face >= 0:
-            raise ValueError(f"{p.name}: the batched path has no boundary source")
+    """A device batch of the problems."""
     return dg3d.Batch([batch_solve(p) for p in problems], precision=precision)

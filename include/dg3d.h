@@ -209,10 +209,12 @@ int dg3d_last_step_timing(dg3d_ctx* ctx, double* avg_stage_ms, int* launches);
  * would hold (the reference builds one such block per MPI rank, Grid.cpp:176-375), advanced
  * together by LSRK54: per stage, one persistent launch per order present covers all solves of
  * that order (the launches run as parallel graph branches), whole steps are replayed from HIP
- * graphs, and elements per workgroup are autotuned per order. The first stage of the first dg3d_batch_step reads the halo traces given
- * here (as dg3d_set_halo on register 0 before dg3d_step); later stages see zero halos. No
- * boundary source. Each solve follows the trajectory of dg3d_step on a context of its own with
- * the same precision (same per-node arithmetic). */
+ * graphs, and elements per workgroup are autotuned per order. The first stage of the first
+ * dg3d_batch_step reads the halo traces given here (as dg3d_set_halo on register 0 before
+ * dg3d_step); later stages see zero halos. A solve may carry a boundary source (as
+ * dg3d_set_source), evaluated on the solve's own clock from t = 0. Each solve follows the
+ * trajectory of dg3d_step on a context of its own with the same precision (same per-node
+ * arithmetic). */
 typedef struct dg3d_batch dg3d_batch;
 typedef struct dg3d_solve_desc {
   int order;                 /* 1..7 */
@@ -228,6 +230,11 @@ typedef struct dg3d_solve_desc {
   const double* state;       /* initial rho', u, v, w: [4][E][(order+1)^3], as dg3d_set_state */
   const double* halo[6];     /* first-stage halo traces per face (as dg3d_set_halo), or NULL */
   double dt;                 /* time step of this solve */
+  /* boundary tone burst on one non-periodic face, as dg3d_set_source (source_face -1: none);
+   * the solve's time starts at 0 */
+  int source_face;
+  double source_amplitude, source_frequency, source_phase, source_t0, source_tau;
+  const double* source_lead; /* per face node delay [faceElem][n*n], or NULL */
 } dg3d_solve_desc;
 
 int dg3d_batch_create(int count, const dg3d_solve_desc* solves, int precision, dg3d_batch** out);

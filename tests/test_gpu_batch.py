@@ -150,3 +150,67 @@ def test_batch_rejects_bad_input():
     d["order"] = 9
     with pytest.raises(dg3d.Dg3dError):
         dg3d.Batch([d])
+
+
+def _with_source(p, face, lead=False):
+    """The tone burst of the 64^3 workload on one face of a pulse problem (the sponge stays)."""
+    f0 = 2.0e5
+    tau = 2.0 / f0
+    p.source = problem.Source(face=face, amplitude=2.0e5, frequency=f0, phase=problem.seeded_phase(face),
+                              t0=0.2 * tau, tau=tau)
+    if lead:
+        a = problem.FACE_AXIS[face]
+        b, c = (1, 2) if a == 0 else ((0, 2) if a == 1 else (0, 1))
+        rng = np.random.default_rng(face)
+        p.source.lead = 1e-6 * rng.random(p.num_elements[b] * p.num_elements[c] * p.n ** 2)
+    return p
+
+
+@pytest.mark.parametrize("precision", [dg3d.FP64, dg3d.MIXED])
+def test_batch_sources_match_single_contexts(precision):
+    """Solves with a boundary source (several faces, one with per-node delays, with and without
+    initial halos) follow dg3d_step on contexts of their own bit for bit, over two calls."""
+    print(f"batch_sources precision={precision}", flush=True)
+    ps = [
+        _with_source(pulse_problem(3, (4, 4, 5), ba=5.0), 0),
+        _with_source(pulse_problem(2, (5, 3, 4), hetero_seed=8), 5, lead=True),
+        _with_source(pulse_problem(4, (3, 4, 3)), 2),
+        pulse_problem(3, (3, 3, 3)),
+        _with_source(pulse_problem(5, (3, 3, 2), periodic=(1, 0, 0)), 1, lead=True),
+    ]
+    b = problem.make_batch(ps, precision=precision)
+    b.step(3)
+    b.step(4)
+    for i, p in enumerate(ps):
+        ref = single_run(p, precision, 7)
+        got = b.get_state(i)
+        assert np.linalg.norm(ref) > 0
+        assert np.array_equal(got, ref), (i, p.name, rel_l2(got, ref))
+    b.close()
+    # the source is not negligible: without it the first solve ends elsewhere
+    plain = single_run(pulse_problem(3, (4, 4, 5), ba=5.0), precision, 7)
+    assert rel_l2(plain, single_run(ps[0], precision, 7)).max() > 1e-6
+
+
+def test_batch_sources_fp64_vs_oracle():
+    """Sourced solves of a batch against the oracle's LSRK54 with the same burst, 1e-12 in FP64."""
+    print("batch_sources_fp64_vs_oracle", flush=True)
+    ps = [_with_source(pulse_problem(2, (4, 3, 5), hetero_seed=7), 3, lead=True),
+          _with_source(pulse_problem(4, (3, 3, 4), ba=5.0), 0),
+          pulse_problem(3, (3, 3, 3))]
+    steps = 12
+    b = problem.make_batch(ps, precision=dg3d.FP64)
+    b.step(steps)
+    for i, p in enumerate(ps):
+        o = Oracle(p, literal=False, num_registers=4)
+        load_problem_into(o, p, 0)
+        o.set_time(0.0)
+        dt = problem.cfl_dt_host(p)
+        for _ in range(steps):
+            o.lsrk54(dt, 0, 1, 2)
+        r = o.get_state(0)
+        err = rel_l2(b.get_state(i), r)
+        print(f"  solve {i} {p.name}: {err.max():.2e}", flush=True)
+        assert np.linalg.norm(r) > 0
+        assert err.max() < 1e-12, (p.name, err)
+    b.close()
