@@ -182,6 +182,24 @@ __device__ __forceinline__ void split3(float x, short& h, short& m, short& l) {
   l = __builtin_bit_cast(short, (__bf16)r2);
 }
 
+typedef float float2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
+
+// split3 of two values at once: each conversion fills both halves of one register, in the
+// operand's order (element 0 in the low half), with the same bits as split3 per element.
+__device__ __forceinline__ void split3x2(float2v x, bf16x2v& h, bf16x2v& m, bf16x2v& l) {
+  h = __builtin_convertvector(x, bf16x2v);
+  const float2v r1 = x - __builtin_convertvector(h, float2v);
+  m = __builtin_convertvector(r1, bf16x2v);
+  const float2v r2 = r1 - __builtin_convertvector(m, float2v);
+  l = __builtin_convertvector(r2, bf16x2v);
+}
+
+__device__ __forceinline__ bf16x4 join4(bf16x2v a, bf16x2v b) {
+  return __builtin_bit_cast(bf16x4, __builtin_shufflevector(a, b, 0, 1, 2, 3));
+}
+
 // Line of a contraction column: a column is (element, axis, field, face node) with the field
 // P/rho0 (f=0) or m_axis (f=1). fo/oo: LDS offset of the line's first node in sF and in sOut,
 // ls: node stride along the line.
@@ -200,114 +218,99 @@ __device__ __forceinline__ void column_line(int col, int& fo, int& oo, int& ls) 
   oo = (le * 6 + af) * NP + base;
 }
 
-// The P = 16/NN columns a lane touches in one tile, one per diagonal block (columns col0 +
-// 16 b), mapped to their lines once per tile. Row k of the tile belongs to block k / NN.
+// One contraction column's line, mapped once per tile: loads and stores of its node m, which
+// read as 0 and are dropped past the line's end (m >= NN) and past the last column.
 template <int NN>
-struct TileColumns {
-  static constexpr int P = 16 / NN;
-  int fo[P], oo[P], ls[P];
-  bool ok[P];
-  __device__ __forceinline__ TileColumns(int col0, int C) {
-#pragma unroll
-    for (int b = 0; b < P; ++b) {
-      const int col = col0 + 16 * b;
-      ok[b] = col < C;
-      fo[b] = oo[b] = ls[b] = 0;
-      if (ok[b]) column_line<NN>(col, fo[b], oo[b], ls[b]);
-    }
+struct TileLine {
+  int fo, oo, ls;
+  bool ok;
+  __device__ __forceinline__ TileLine(int col, int C) {
+    ok = col < C;
+    column_line<NN>(ok ? col : 0, fo, oo, ls);  // branch-free: column 0 stands in past the end
   }
-  // the line of row k: one LDS access, its offset selected among the P columns
-  __device__ __forceinline__ bool pick(int k, const int* base, int& off) const {
-    const int b = k / NN, m = k - b * NN;
-    int o = 0, st = 0;
-    bool in = false;
-#pragma unroll
-    for (int bb = 0; bb < P; ++bb) {
-      o = (b == bb) ? base[bb] : o;
-      st = (b == bb) ? ls[bb] : st;
-      in = (b == bb) ? ok[bb] : in;
-    }
-    off = o + m * st;
-    return in;
-  }
-  // operand of row k (0 outside the blocks and past the last column)
-  __device__ __forceinline__ float load(const float* sF, int k) const {
-    int off;
-    const bool in = pick(k, fo, off);
-    const float v = sF[in ? off : 0];
+  __device__ __forceinline__ float load(const float* sF, int m) const {
+    const bool in = ok && m < NN;
+    const float v = sF[in ? fo + m * ls : 0];
     return in ? v : 0.f;
   }
-  __device__ __forceinline__ void store(float* sOut, int i, float v) const {
-    int off;
-    if (pick(i, oo, off)) sOut[off] = v;
+  __device__ __forceinline__ void store(float* sOut, int m, float v) const {
+    if (ok && m < NN) sOut[oo + m * ls] = v;
   }
 };
 
 // The workgroup's derivative contractions as one GEMM on the matrix cores:
 // out[i][col] = sum_m A[i][m] line_col[m], one column per (element, axis, field, face node).
-// P = 16/NN copies of A sit block-diagonally in the 16x16 operand, so one tile covers 16*P
-// columns (16 per diagonal block). Each wave takes tiles round-robin.
-//   CONTRACT 1: v_mfma_f32_16x16x4f32, FP32 operands.
+// P copies of A sit block-diagonally in the 16x16 operand, block b on rows and K indices
+// [b*BS, b*BS + NN); the block stride BS (4 for NN <= 4, else 8) is a multiple of 4, so the 4
+// rows a lane holds (4g..4g+3, in both the K operand and the result) lie in one block. A tile
+// covers 16 columns per block (columns col0 + 16 b), and waves take tiles round-robin.
+//   CONTRACT 1: v_mfma_f32_16x16x4f32, FP32 operands, 4 K steps.
 //   CONTRACT 2: v_mfma_f32_16x16x16bf16_1k with A and the lines split into three bf16 pieces
 //               each; the six products of combined order <= 2 are accumulated in FP32,
 //               smallest first.
 // Lane layouts (16x16 tile, lane l, g = l/16, j = l%16):
-//   16x16x4f32:        A[j][k = g], B[k = g][j], D[4g + r][j]
+//   16x16x4f32:        A[j][k = g], B[k = g][j], D[4g + r][j]   (K step s covers 4s..4s+3)
 //   16x16x16bf16_1k:   A[j][k = 4g + t], B[k = 4g + t][j], D[4g + r][j]
 template <int NN, int EPB, int NT, int CONTRACT>
 __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf, float* sOut, int tid) {
-  constexpr int P = 16 / NN;
+  constexpr int BS = NN <= 4 ? 4 : 8;
+  constexpr int P = 16 / BS;
   constexpr int C = EPB * 6 * NN * NN;
   constexpr int TC = 16 * P;
   constexpr int T = (C + TC - 1) / TC;
   constexpr int W = NT / 64;
   static_assert(NT % 64 == 0, "whole waves");
+  static_assert(NN <= 8, "one block per 8 rows");
   const int lane = tid & 63, wave = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
+  const int bj = j / BS, mj = j - bj * BS;    // block and line node of operand row j
+  const int bd = (4 * g) / BS;                // block of the lane's result rows 4g..4g+3
+  const int md = 4 * g - bd * BS;             // their first line node
   if constexpr (CONTRACT == 1) {
-    constexpr int KS = (P * NN + 3) / 4;
-    float aop[KS];
+    float aop[4];
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      const int k = 4 * s + g;
-      const bool in = (j / NN == k / NN) && (j / NN < P) && (k < P * NN);
-      aop[s] = in ? sAf[(j % NN) * NN + (k % NN)] : 0.f;
+    for (int s = 0; s < 4; ++s) {
+      const int bk = (4 * s) / BS, mk = 4 * s + g - bk * BS;
+      aop[s] = (bj == bk && mj < NN && mk < NN) ? sAf[mj * NN + mk] : 0.f;
     }
     for (int t = wave; t < T; t += W) {
-      TileColumns<NN> tc(t * TC + j, C);
+      // K step s reads block (4s)/BS for every lane; the results go to block bd, whose line is
+      // selected among the P mapped ones
       floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      TileLine<NN> out(t * TC + j, C);
 #pragma unroll
-      for (int s = 0; s < KS; ++s) {
-        const int k = 4 * s + g;
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aop[s], tc.load(sF, k), acc, 0, 0, 0);
+      for (int b = 0; b < P; ++b) {
+        const TileLine<NN> ln = b == 0 ? out : TileLine<NN>(t * TC + 16 * b + j, C);
+#pragma unroll
+        for (int s = b * BS / 4; s < (b + 1) * BS / 4; ++s)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aop[s], ln.load(sF, 4 * s + g - b * BS), acc, 0, 0, 0);
+        if (b > 0) {
+          out.fo = bd == b ? ln.fo : out.fo;
+          out.oo = bd == b ? ln.oo : out.oo;
+          out.ls = bd == b ? ln.ls : out.ls;
+          out.ok = bd == b ? ln.ok : out.ok;
+        }
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) tc.store(sOut, 4 * g + r, acc[r]);
+      for (int r = 0; r < 4; ++r) out.store(sOut, md + r, acc[r]);
     }
   } else {
     bf16x4 ah, am, al;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int k = 4 * g + q;
-      const bool in = (j / NN == k / NN) && (j / NN < P) && (k < P * NN);
+      const int mk = md + q;
       short h, m, l;
-      split3(in ? sAf[(j % NN) * NN + (k % NN)] : 0.f, h, m, l);
+      split3((bj == bd && mj < NN && mk < NN) ? sAf[mj * NN + mk] : 0.f, h, m, l);
       ah[q] = h;
       am[q] = m;
       al[q] = l;
     }
     for (int t = wave; t < T; t += W) {
-      TileColumns<NN> tc(t * TC + j, C);
-      bf16x4 bh, bm, bl;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float bv = tc.load(sF, 4 * g + q);
-        short h, m, l;
-        split3(bv, h, m, l);
-        bh[q] = h;
-        bm[q] = m;
-        bl[q] = l;
-      }
+      const TileLine<NN> ln(t * TC + 16 * bd + j, C);
+      bf16x2v h0, m0, l0, h1, m1, l1;
+      split3x2(float2v{ln.load(sF, md), ln.load(sF, md + 1)}, h0, m0, l0);
+      split3x2(float2v{ln.load(sF, md + 2), ln.load(sF, md + 3)}, h1, m1, l1);
+      const bf16x4 bh = join4(h0, h1), bm = join4(m0, m1), bl = join4(l0, l1);
       floatx4 acc = {0.f, 0.f, 0.f, 0.f};
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bm, acc, 0, 0, 0);
@@ -316,7 +319,7 @@ __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf,
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bh, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) tc.store(sOut, 4 * g + r, acc[r]);
+      for (int r = 0; r < 4; ++r) ln.store(sOut, md + r, acc[r]);
     }
   }
 }
