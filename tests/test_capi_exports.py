@@ -132,3 +132,26 @@ def test_create_rejects_bad_descriptors_on_the_host():
     assert rc == dg3d.E_INVALID and b"num_elements" in msg
     rc, msg = create(desc(regs=0))
     assert rc == dg3d.E_INVALID and b"num_registers" in msg
+
+
+def test_batch_rejects_bad_solves_on_the_host():
+    """dg3d_batch_create checks every solve description before any device call: time step,
+    order, and a boundary source's face, periodicity and envelope width."""
+    from helpers import problem
+    p = problem.config_c1()
+    src = dict(face=0, amplitude=1e3, frequency=1e5, phase=0.0, t0=0.0, tau=1e-5)
+
+    def rejected(**changes):
+        d = problem.batch_solve(p)
+        d.update(changes)
+        try:
+            dg3d.Batch([d])
+        except dg3d.Dg3dError as e:
+            return e.args
+        raise AssertionError(f"accepted {changes}")
+
+    assert "dt must be positive" in str(rejected(dt=-1.0))
+    assert "order must be 1..7" in str(rejected(order=9))
+    assert "source face must be 0..5" in str(rejected(source=dict(src, face=6)))
+    assert "source on a periodic face" in str(rejected(source=dict(src, face=4), periodic=(1, 0, 0)))
+    assert "source tau must be positive" in str(rejected(source=dict(src, tau=0.0)))
