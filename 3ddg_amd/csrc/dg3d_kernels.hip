@@ -742,9 +742,18 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   }
 }
 
+// Waves per SIMD the register allocation must allow. The FP64 stage needs more than the 80
+// registers a target of 6 leaves when the RK register is loaded early, or at 7 and 8 nodes per
+// line, and would spill to scratch; those shapes target 5 (96 registers).
+template <int NN, typename CT, int MINW, bool LATE>
+constexpr int waves_target() {
+  return (MINW > 5 && sizeof(CT) == 8 && (!LATE || NN >= 7)) ? 5 : MINW;
+}
+
 template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT, int CONTRACT = 0, bool LATE = false,
           bool LDG = false>
-__global__ void __launch_bounds__((Geo<NN, EPB>::NT), MINW) k_rhs(const RhsLaunch L, const GroupMap gm) {
+__global__ void __launch_bounds__((Geo<NN, EPB>::NT), (waves_target<NN, CT, MINW, LATE>())) k_rhs(
+    const RhsLaunch L, const GroupMap gm) {
   __shared__ StageSmem<NN, EPB, CT, CONTRACT, LDG> sm;
   const uint32_t grp = map_group(blockIdx.x, gm);
   stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI, DUT, CONTRACT, LATE, LDG>(L, L.mesh, L.ext.trace, (int)(grp * EPB),
