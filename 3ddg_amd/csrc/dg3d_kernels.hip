@@ -167,6 +167,16 @@ __device__ __forceinline__ void face_coeffs(int side, CT c1, CT r1, CT ir1, CT c
   K[11] = mass_k11(ba2);
 }
 
+// A value known to be equal on every lane, as such to the compiler (bit-exact: the two halves
+// of the double are moved separately)
+__device__ __forceinline__ double wave_uniform(double v) {
+  const u32x2 u = __builtin_bit_cast(u32x2, v);
+  u32x2 r;
+  r[0] = __builtin_amdgcn_readfirstlane(u[0]);
+  r[1] = __builtin_amdgcn_readfirstlane(u[1]);
+  return __builtin_bit_cast(double, r);
+}
+
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 
@@ -437,10 +447,26 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   };
   if constexpr (!LATE) load_pre();
   const Rsrc rm = make_rsrc(m.mat, (uint32_t)(m.E * kMatStride * 8));
-  const uint32_t mi = (uint32_t)(valid ? e : 0) * kMatStride;
-  const CT c = (CT)bload<double>(rm, mi + MAT_C), r0 = (CT)bload<double>(rm, mi + MAT_RHO0);
-  const CT ba = (CT)bload<double>(rm, mi + MAT_BA), sg = (CT)bload<double>(rm, mi + MAT_SIGMA);
-  const CT ir0 = (CT)bload<double>(rm, mi + MAT_IRHO0);
+  // the lane's element material. When the element slots fill whole waves (ES a multiple of 64)
+  // the element is the same on every lane of a wave, and its record is read once per wave
+  // through the scalar cache, which keeps these loads off the vector memory pipeline that the
+  // state and the neighbour gathers use
+  double mat[6];
+  if constexpr (ES % 64 == 0) {
+    typedef const __attribute__((address_space(4))) double* ConstPtr;  // read-only in the kernel
+    const int lw = __builtin_amdgcn_readfirstlane(le);
+    const ConstPtr pm = (ConstPtr)(m.mat + (size_t)((lw < EPB && e0 + lw < E) ? e0 + lw : 0) * kMatStride);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) mat[k] = (k < 5 || LDG) ? wave_uniform(pm[k]) : 0.0;
+  } else {
+    const uint32_t mi = (uint32_t)(valid ? e : 0) * kMatStride;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) mat[k] = bload<double>(rm, mi + k);
+    mat[MAT_DIFF] = LDG ? bload<double>(rm, mi + MAT_DIFF) : 0.0;
+  }
+  const CT c = (CT)mat[MAT_C], r0 = (CT)mat[MAT_RHO0];
+  const CT ba = (CT)mat[MAT_BA], sg = (CT)mat[MAT_SIGMA];
+  const CT ir0 = (CT)mat[MAT_IRHO0];
   Rsrc rqq[3];
   double qq[3] = {0.0, 0.0, 0.0};
   CT dcoef = CT(0);
@@ -450,7 +476,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
       rqq[a] = make_rsrc(L.q_in + (4 + a) * fs, fb64);
       qq[a] = bload<double>(rqq[a], gi);
     }
-    dcoef = (CT)bload<double>(rm, mi + MAT_DIFF);
+    dcoef = (CT)mat[MAT_DIFF];
   }
 
   // helper lanes, one per (element, face): material of both sides and 1/width. These and the
