@@ -169,9 +169,13 @@ def second_harmonic(trace, dt, f0):
     return float(np.abs(np.sum(trace * np.exp(-2j * np.pi * 2 * f0 * t))) * 2.0 / max(len(trace), 1))
 
 
-def compare_precisions(pk, p, steps, warmup, mixed_contraction=0, matrix_core_runs=True):
+CONTRACTION_NAMES = ["valu", "mfma_f32", "mfma_bf16x3"]
+
+
+def compare_precisions(pk, p, steps, warmup, mixed_contraction=0, other_contractions=True):
     """FP64 and mixed runs of the same problem: speed and accuracy of mixed at t_end (the mixed
-    run's derivative contraction: 0 VALU, 1 MFMA FP32, 2 MFMA split-BF16)."""
+    run's derivative contraction: 0 VALU, 1 MFMA FP32, 2 MFMA split-BF16), and optionally the
+    speed and accuracy of the other two contractions."""
     pr = probe_node(p)
     ctx64, r64 = timed_run(pk, p, pk.dg3d.FP64, steps, warmup, probe=pr)
     s64 = ctx64.get_state(0)
@@ -194,19 +198,20 @@ def compare_precisions(pk, p, steps, warmup, mixed_contraction=0, matrix_core_ru
     h32 = second_harmonic(tr32, r32["dt"], f0)
     rate64 = p.dof * 5 * steps / r64["elapsed"]
     rate32 = p.dof * 5 * steps / r32["elapsed"]
-    matrix_core = {}
-    for name, con in (("mfma_f32", pk.dg3d.CONTRACT_MFMA_F32), ("mfma_bf16x3", pk.dg3d.CONTRACT_MFMA_BF16)):
-        if not matrix_core_runs:
-            break
+    others = {}
+    for con in range(len(CONTRACTION_NAMES)):
+        if not other_contractions or con == mixed_contraction:
+            continue
         ctxm, rm = timed_run(pk, p, pk.dg3d.MIXED, steps, warmup, contraction=con)
         sm = ctxm.get_state(0).reshape(4, -1)
         ctxm.close()
         relm = [float(np.linalg.norm(sm[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-300)) for f in range(4)]
-        matrix_core[name] = dict(stage_ms=rm["stage_ms"], variant=[rm["variant"], rm["xcd"]],
-                                 speedup_vs_fp64=r64["stage_ms"] / rm["stage_ms"], rel_l2_max=max(relm))
+        others[CONTRACTION_NAMES[con]] = dict(stage_ms=rm["stage_ms"], variant=[rm["variant"], rm["xcd"]],
+                                              speedup_vs_fp64=r64["stage_ms"] / rm["stage_ms"],
+                                              rel_l2_max=max(relm))
     return dict(
         workload=p.name, elements=list(p.num_elements), order=p.order, dof=p.dof, steps=steps,
-        mixed_contraction=["valu", "mfma_f32", "mfma_bf16x3"][mixed_contraction],
+        mixed_contraction=CONTRACTION_NAMES[mixed_contraction],
         fp64_dof_stage_per_s=rate64, mixed_dof_stage_per_s=rate32, mixed_speedup=rate32 / rate64,
         fp64_stage_ms=r64["stage_ms"], mixed_stage_ms=r32["stage_ms"],
         fp64_variant=[r64["variant"], r64["xcd"]], mixed_variant=[r32["variant"], r32["xcd"]],
@@ -215,7 +220,7 @@ def compare_precisions(pk, p, steps, warmup, mixed_contraction=0, matrix_core_ru
         mixed_rel_l2_max=max(rel), mixed_rel_l2=rel,
         peak_p_fp64=pk64, peak_p_rel_err=abs(pk32 - pk64) / max(pk64, 1e-300),
         second_harmonic_fp64=h64, second_harmonic_rel_err=abs(h32 - h64) / max(h64, 1e-300),
-        matrix_core_contraction=matrix_core,
+        other_contractions=others,
     )
 
 
@@ -403,8 +408,9 @@ def main():
     if not args.no_extra:
         out["mixed_c2"] = compare_precisions(pk, pk.problem.config_c2(num_elements=args.elements, steps=args.steps),
                                              args.steps, args.warmup)
+        # the N=7 precision workload runs its derivative contraction as split-BF16 MFMA GEMMs
         out["mixed_c3"] = compare_precisions(pk, pk.problem.config_c3(num_elements=args.elements, steps=args.steps),
-                                             args.steps, args.warmup)
+                                             args.steps, args.warmup, mixed_contraction=pk.dg3d.CONTRACT_MFMA_BF16)
     if not args.no_extra:
         out["small_c1"] = small_block_run(pk)
         out["other_paths_c2"] = other_paths_run(pk, args.elements, min(args.steps, 200), args.warmup)
@@ -415,7 +421,7 @@ def main():
         # split-BF16 matrix-core contraction against FP64
         out["combined_c5"] = compare_precisions(pk, pk.problem.config_c5(steps=args.c5_steps), args.c5_steps,
                                                 args.warmup, mixed_contraction=pk.dg3d.CONTRACT_MFMA_BF16,
-                                                matrix_core_runs=False)
+                                                other_contractions=False)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pk.problem.config_c2(num_elements=args.elements), args.cpu_steps)
         out["cpu_literal_rk33"] = cpu_literal_baseline(pk.problem.config_c2(num_elements=args.elements),
