@@ -482,8 +482,13 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   // helper lanes, one per (element, face): material of both sides and 1/width. These and the
   // line operators are issued before the neighbour gathers: phase 1 needs them, and the wait
   // for them before the first barrier then leaves the gathers in flight until phase 2
-  const bool helper = tid < EPB * 6;
-  const int hle = tid / 6, hdir = tid - (tid / 6) * 6;
+  // The helper lanes sit in the workgroup's last wave and the operator lanes in its second, when
+  // they fit there, so that no wave issues both sets of loads on top of its own
+  constexpr int HB = ((NT / 64 - 1) * 64 + EPB * 6 <= NT) ? (NT / 64 - 1) * 64 : 0;
+  constexpr int OB = (NT >= 128 && 64 + NN * NN <= NT) ? 64 : 0;
+  const int ht = tid - HB, ot = tid - OB;
+  const bool helper = ht >= 0 && ht < EPB * 6;
+  const int hle = ht / 6, hdir = ht - (ht / 6) * 6;
   const int he = e0 + hle;
   double ms_c = 1.0, ms_r = 1.0, ms_ir = 1.0, mo_c = 1.0, mo_r = 1.0, mo_ba = 0.0, mo_ir = 1.0, hiw = 0.0;
   if (helper && he < E) {
@@ -504,17 +509,17 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     if ((hdir & 1) == 0) hiw = sel3(axis, m.inv_w[0], m.inv_w[1], m.inv_w[2])[sel3(axis, ec[0], ec[1], ec[2])];
   }
   CT av = CT(0), l0v = CT(0), lnv = CT(0);
-  if (tid < NN * NN) {
-    if constexpr (std::is_same<CT, double>::value) av = L.opd[0][tid];
-    else av = L.opf[0][tid];
+  if (ot >= 0 && ot < NN * NN) {
+    if constexpr (std::is_same<CT, double>::value) av = L.opd[0][ot];
+    else av = L.opf[0][ot];
   }
-  if (tid < NN) {
+  if (ot >= 0 && ot < NN) {
     if constexpr (std::is_same<CT, double>::value) {
-      l0v = L.opd[1][tid];
-      lnv = L.opd[2][tid];
+      l0v = L.opd[1][ot];
+      lnv = L.opd[2][ot];
     } else {
-      l0v = L.opf[1][tid];
-      lnv = L.opf[2][tid];
+      l0v = L.opf[1][ot];
+      lnv = L.opf[2][ot];
     }
   }
 
@@ -605,12 +610,12 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
       if ((hdir & 1) == 0) sIw[hle * 3 + 2 - (hdir >> 1)] = (CT)hiw;
     }
 #pragma unroll
-    for (int k = 0; k < kNK; ++k) sK[tid * kNK + k] = K[k];
+    for (int k = 0; k < kNK; ++k) sK[ht * kNK + k] = K[k];
   }
-  if (tid < NN * NN) sA[tid] = av;
-  if (tid < NN) {
-    sL0[tid] = l0v;
-    sLN[tid] = lnv;
+  if (ot >= 0 && ot < NN * NN) sA[ot] = av;
+  if (ot >= 0 && ot < NN) {
+    sL0[ot] = l0v;
+    sLN[ot] = lnv;
   }
   __syncthreads();
 
