@@ -447,12 +447,14 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   };
   if constexpr (!LATE) load_pre();
   const Rsrc rm = make_rsrc(m.mat, (uint32_t)(m.E * kMatStride * 8));
-  // the lane's element material. When the element slots fill whole waves (ES a multiple of 64)
-  // the element is the same on every lane of a wave, and its record is read once per wave
-  // through the scalar cache, which keeps these loads off the vector memory pipeline that the
-  // state and the neighbour gathers use
+  // the lane's element material. When the element slots fill whole waves (ES a multiple of 64),
+  // or the workgroup has one element, the element is the same on every lane of a wave that
+  // holds any of its slots (lanes past the workgroup's elements return before the epilogue),
+  // and its record is read once per wave through the scalar cache, which keeps these loads off
+  // the vector memory pipeline that the state and the neighbour gathers use. The one-element
+  // shape of 7 nodes per line keeps the vector loads: it ran 5 % slower with the scalar ones
   double mat[6];
-  if constexpr (ES % 64 == 0) {
+  if constexpr (ES % 64 == 0 || (EPB == 1 && NN != 7)) {
     typedef const __attribute__((address_space(4))) double* ConstPtr;  // read-only in the kernel
     const int lw = __builtin_amdgcn_readfirstlane(le);
     const ConstPtr pm = (ConstPtr)(m.mat + (size_t)((lw < EPB && e0 + lw < E) ? e0 + lw : 0) * kMatStride);
