@@ -138,7 +138,8 @@ def pkg_grid(ctx, reg):
 
 
 def test_lsrk54_c1_full_run():
-    """configs[0] workload on the GPU: 8^3, N=3, 100 LSRK54 steps vs the oracle."""
+    """The reference's CPU-sized workload on the GPU: 8^3 elements, N=3, Gaussian pulse, 100
+    LSRK54 steps vs the oracle."""
     print("lsrk54_c1_full_run", flush=True)
     p = small_c1(steps=100)
     g, r, ctx, o = _run_both(p, dg3d.LSRK54, p.steps)
