@@ -1012,14 +1012,19 @@ __global__ void __launch_bounds__((Geo<NN, Shape<NN>::EPB>::NT)) k_ldg_q(const L
       }
     }
   }
+  // operators on the second wave and 1/width on the last, when they fit there, so that wave 0
+  // issues only its own loads, as in the stage kernel
+  constexpr int HB = ((NT / 64 - 1) * 64 + EPB * 3 <= NT) ? (NT / 64 - 1) * 64 : 0;
+  constexpr int OB = (NT >= 128 && 64 + NN * NN <= NT) ? 64 : 0;
+  const int ht = tid - HB, ot = tid - OB;
   double av = 0.0, l0v = 0.0, lnv = 0.0, iwv = 0.0;
-  if (tid < NN * NN) av = L.opd[0][tid];
-  if (tid < NN) {
-    l0v = L.opd[1][tid];
-    lnv = L.opd[2][tid];
+  if (ot >= 0 && ot < NN * NN) av = L.opd[0][ot];
+  if (ot >= 0 && ot < NN) {
+    l0v = L.opd[1][ot];
+    lnv = L.opd[2][ot];
   }
-  if (tid < EPB * 3) {
-    const int he = e0 + tid / 3, a = tid - (tid / 3) * 3;
+  if (ht >= 0 && ht < EPB * 3) {
+    const int he = e0 + ht / 3, a = ht - (ht / 3) * 3;
     if (he < E) {
       int ec[3];
       elem_coords(m, (uint32_t)he, ec);
@@ -1029,12 +1034,12 @@ __global__ void __launch_bounds__((Geo<NN, Shape<NN>::EPB>::NT)) k_ldg_q(const L
 
   // ---- phase 1: the workgroup's rho', operators and 1/width into LDS ----------------------
   if (valid) sV[le * NP + node] = v;
-  if (tid < NN * NN) sA[tid] = av;
-  if (tid < NN) {
-    sL0[tid] = l0v;
-    sLN[tid] = lnv;
+  if (ot >= 0 && ot < NN * NN) sA[ot] = av;
+  if (ot >= 0 && ot < NN) {
+    sL0[ot] = l0v;
+    sLN[ot] = lnv;
   }
-  if (tid < EPB * 3) sIw[tid] = iwv;
+  if (ht >= 0 && ht < EPB * 3) sIw[ht] = iwv;
   __syncthreads();
 
   // ---- phase 2: central fluxes of the face nodes -----------------------------------------
