@@ -419,7 +419,10 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   const int e = e0 + le;
   const bool io = (le < EPB) && (e < E);
   const bool valid = io && node < NP;
-  const uint32_t gi = (uint32_t)(io ? e : 0) * ES + node;
+  // lanes past the workgroup's elements load a slot of its last element, a line that its own
+  // lanes read anyway, rather than one fixed element that every workgroup would read
+  const int ge = io ? e : min(e0 + EPB - 1, E - 1);
+  const uint32_t gi = (uint32_t)ge * ES + node;
   const uint32_t fb64 = (uint32_t)(fs * 8), fbdu = (uint32_t)(fs * sizeof(DUT));
   Rsrc rq[4], rp[4];
 #pragma unroll
@@ -461,7 +464,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
 #pragma unroll
     for (int k = 0; k < 6; ++k) mat[k] = (k < 5 || LDG) ? wave_uniform(pm[k]) : 0.0;
   } else {
-    const uint32_t mi = (uint32_t)(valid ? e : 0) * kMatStride;
+    const uint32_t mi = (uint32_t)ge * kMatStride;  // padding slots read their element's record
 #pragma unroll
     for (int k = 0; k < 5; ++k) mat[k] = bload<double>(rm, mi + k);
     mat[MAT_DIFF] = LDG ? bload<double>(rm, mi + MAT_DIFF) : 0.0;
