@@ -1,0 +1,125 @@
+"""The matrix-core contraction's tile mapping (contract_mfma in 3ddg_amd/csrc/dg3d_kernels.hip),
+emulated on the CPU for every stage shape the library builds.
+
+The emulation restates the kernel's index arithmetic: the column -> line map (column_line), the
+aligned diagonal blocks (one every 4 rows for n <= 4, every 8 rows for n >= 5), the lane layouts of
+v_mfma_f32_16x16x4f32 and v_mfma_f32_16x16x16bf16_1k, and the round-robin of tiles over waves.
+It checks what the kernel relies on without a bounds check of its own: every LDS read and write
+stays inside its array, every output line is written exactly once, and the tiled GEMM equals
+the direct per-node contraction out[i] = sum_m A[i][m] line[m] along each axis.
+"""
+import numpy as np
+import pytest
+
+# stage shapes: elements per workgroup (both autotuner shapes) per nodes-per-line n
+SHAPES = {2: (32, 16), 3: (8, 16), 4: (4, 8), 5: (2, 4), 6: (2, 1), 7: (2, 1), 8: (1, 2)}
+
+
+def elem_stride(n):
+    """Node slots per element (128-byte lines of doubles), as dg3d_device.hpp's elem_stride."""
+    return n ** 3 if (n == 2 or n ** 3 % 32 == 0) else (n ** 3 + 31) // 32 * 32
+
+
+def column_line(n, col):
+    """(element, axis, field, face node) of a column -> first node offsets in sF and sOut, and
+    the node stride along the line."""
+    NP, NF2 = n ** 3, n * n
+    le, rr = divmod(col, 6 * NF2)
+    af, fn = divmod(rr, NF2)
+    a, f = af >> 1, af & 1
+    b, c = divmod(fn, n)
+    base = b * n + c if a == 0 else (b * NF2 + c if a == 1 else b * NF2 + c * n)
+    ls = NF2 if a == 0 else (n if a == 1 else 1)
+    return le * 4 * NP + (0 if f == 0 else 1 + a) * NP + base, (le * 6 + af) * NP + base, ls
+
+
+def emulate(n, epb, mode, A, sF):
+    NP, NF2 = n ** 3, n * n
+    NT = (epb * elem_stride(n) + 63) // 64 * 64
+    W = NT // 64
+    BS = 4 if n <= 4 else 8
+    P = 16 // BS
+    C = epb * 6 * NF2
+    TC = 16 * P
+    T = (C + TC - 1) // TC
+    sOut = np.full(epb * 6 * NP, np.nan)
+    writes = np.zeros(epb * 6 * NP, int)
+
+    def line(col):
+        ok = col < C
+        fo, oo, ls = column_line(n, col if ok else 0)
+        return ok, fo, oo, ls
+
+    def load(ln, m):
+        ok, fo, _, ls = ln
+        if ok and m < n:
+            i = fo + m * ls
+            assert 0 <= i < sF.size
+            return sF[i]
+        return 0.0
+
+    for wave in range(W):
+        for t in range(wave, T, W):
+            Aop = np.zeros((16, 16))
+            Bop = np.zeros((16, 16))
+            out = []
+            for lane in range(64):
+                j, g = lane & 15, lane >> 4
+                bj, mj = divmod(j, BS)
+                bd = (4 * g) // BS
+                md = 4 * g - bd * BS
+                if mode == 2:    # K rows 4g..4g+3 of the lane, one block
+                    ln = line(t * TC + 16 * bd + j)
+                    for q in range(4):
+                        k, mk = 4 * g + q, md + q
+                        Aop[j, k] = A[mj, mk] if (bj == bd and mj < n and mk < n) else 0.0
+                        Bop[k, j] = load(ln, mk)
+                    out.append((j, [(4 * g + r, ln, md + r) for r in range(4)]))
+                else:            # K step s covers rows 4s..4s+3, K index 4s + g
+                    lines = [line(t * TC + 16 * b + j) for b in range(P)]
+                    for s in range(4):
+                        kk = 4 * s + g
+                        bk = (4 * s) // BS
+                        mk = kk - bk * BS
+                        assert kk // BS == bk
+                        Aop[j, kk] = A[mj, mk] if (bj == bk and mj < n and mk < n) else 0.0
+                        Bop[kk, j] = load(lines[bk], mk)
+                    out.append((j, [(4 * g + r, lines[bd], md + r) for r in range(4)]))
+            D = Aop @ Bop
+            for j, rows in out:
+                for row, (ok, _, oo, ls), m in rows:
+                    if ok and m < n:
+                        i = oo + m * ls
+                        assert 0 <= i < sOut.size
+                        sOut[i] = D[row, j]
+                        writes[i] += 1
+    return sOut, writes
+
+
+def direct(n, epb, A, sF):
+    NP, NF2 = n ** 3, n * n
+    ref = np.zeros(epb * 6 * NP)
+    for le in range(epb):
+        for node in range(NP):
+            idx = (node // NF2, (node // n) % n, node % n)
+            for a in range(3):
+                ls = NF2 if a == 0 else (n if a == 1 else 1)
+                base = node - idx[a] * ls
+                F = le * 4 * NP
+                ref[(le * 6 + 2 * a) * NP + node] = sum(A[idx[a], m] * sF[F + base + m * ls] for m in range(n))
+                ref[(le * 6 + 2 * a + 1) * NP + node] = sum(
+                    A[idx[a], m] * sF[F + (1 + a) * NP + base + m * ls] for m in range(n))
+    return ref
+
+
+@pytest.mark.parametrize("n", sorted(SHAPES))
+def test_tile_map_covers_every_line_once_in_bounds(n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    for epb in SHAPES[n]:
+        sF = rng.standard_normal(epb * 4 * n ** 3)
+        ref = direct(n, epb, A, sF)
+        for mode in (1, 2):
+            got, writes = emulate(n, epb, mode, A, sF)
+            assert (writes == 1).all(), (n, epb, mode, writes.min(), writes.max())
+            np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
