@@ -85,9 +85,18 @@ def build_driver():
     return out
 
 
+def build_stream_probe():
+    """The streaming-ceiling probe that bench.py runs beside the roofline (a program of its own)."""
+    out = os.path.join(PKG, "dg3d_stream_probe")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", *DEVICE_FLAGS,
+          os.path.join(CSRC, "stream_probe.hip"), "-o", out])
+    return out
+
+
 def build_all(verbose_resources=False):
     lib = build_library(verbose_resources)
     drv = build_driver()
+    build_stream_probe()
     return lib, drv
 
 

@@ -14,6 +14,7 @@ import argparse
 import importlib
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -71,6 +72,25 @@ def calibrate_copy():
     el = (time.perf_counter() - t0) / 20
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copy_ms": el * 1e3,
                       "copy_GBs": 2 * n * 8 / el / 1e9}))
+
+
+STREAM_PROBE = os.path.join(ROOT, "3ddg_amd", "dg3d_stream_probe")
+
+
+def stream_ceiling():
+    """The device's streaming rate for the stage kernel's access pattern (8 FP64 arrays of the
+    64^3 N=4 block's node count read and 8 written, nontemporal, by 256-lane workgroups each
+    streaming one chunk of every array), from 3ddg_amd/dg3d_stream_probe run as a child process.
+    None when the probe is not built or fails."""
+    if not os.path.exists(STREAM_PROBE):
+        return None
+    try:
+        r = subprocess.run([STREAM_PROBE], capture_output=True, text=True, timeout=120)
+    except subprocess.TimeoutExpired:
+        return None
+    if r.returncode != 0 or not r.stdout.strip():
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def timed_run(pk, p, precision, steps, warmup, probe=None, variant=-1, xcd=1, contraction=0, scheme=None,
@@ -406,6 +426,13 @@ def main():
         },
     }
     if not args.no_extra:
+        sc = stream_ceiling()
+        if sc is not None:
+            out["stream_ceiling"] = dict(
+                GBs=sc["stream_GBs"], frac_of_peak=sc["stream_GBs"] / HBM_PEAK_GBS,
+                stage_frac_of_ceiling=achieved / sc["stream_GBs"],
+                pattern=f"{sc['arrays_read']} FP64 arrays read and {sc['arrays_written']} written, nontemporal, "
+                        f"256-lane workgroups streaming {sc['chunk']}-value chunks")
         out["mixed_c2"] = compare_precisions(pk, pk.problem.config_c2(num_elements=args.elements, steps=args.steps),
                                              args.steps, args.warmup)
         # the N=7 precision workload runs its derivative contraction as split-BF16 MFMA GEMMs
