@@ -10,6 +10,7 @@ import numpy as np
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libdg3d.so")
+ABI_VERSION = 1  # the structure layouts and signatures below (DG3D_ABI_VERSION of include/dg3d.h)
 
 OK = 0
 E_INVALID, E_HIP, E_NOMEM, E_NAN, E_UNSUPPORTED, E_IO = -1, -2, -3, -4, -5, -6
@@ -144,6 +145,9 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        if L.dg3d_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{LIB_PATH} has ABI version {L.dg3d_abi_version()}, this binding "
+                               f"describes version {ABI_VERSION}: rebuild the library")
         _lib = L
     return _lib
 

@@ -31,6 +31,8 @@
 extern "C" {
 #endif
 
+/* Incremented whenever a structure layout or an entry point's signature changes; a caller compares
+ * it with dg3d_abi_version() of the library it loads. */
 #define DG3D_ABI_VERSION 1
 
 /* status codes */
