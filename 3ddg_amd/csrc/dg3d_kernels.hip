@@ -135,6 +135,25 @@ __device__ __forceinline__ void bstore(T v, Rsrc r, uint32_t i) {
   }
 }
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// The four FP32 fields of one node of a node-major register at byte offset off, in one 16-byte
+// access (the whole vector is reinterpreted; no swizzle is bit-cast)
+template <int POL = 0>
+__device__ __forceinline__ void bload4f(Rsrc r, uint32_t off, double out[4]) {
+  const f32x4 a = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, POL));
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out[k] = (double)a[k];
+}
+template <int POL = 0>
+__device__ __forceinline__ void bstore4f(const float v[4], Rsrc r, uint32_t off) {
+  f32x4 a;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) a[k] = v[k];
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a), r, off, 0, POL);
+}
+
 template <typename T>
 __device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
 template <typename T>
@@ -423,22 +442,40 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   // lanes read anyway, rather than one fixed element that every workgroup would read
   const int ge = io ? e : min(e0 + EPB - 1, E - 1);
   const uint32_t gi = (uint32_t)ge * ES + node;
-  const uint32_t fb64 = (uint32_t)(fs * 8), fbdu = (uint32_t)(fs * sizeof(DUT));
-  Rsrc rq[4], rp[4];
+  const uint32_t fb64 = (uint32_t)(fs * 8);
+  Rsrc rq[4];
 #pragma unroll
   for (int f = 0; f < 4; ++f) rq[f] = make_rsrc(L.q_in + f * fs, fb64);
   double q[4], pre[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int f = 0; f < 4; ++f) q[f] = bload<double>(rq[f], gi);
+  // The LSRK register is never read across a face, so its layout is free. In FP32 with whole-wave
+  // elements it is node-major (a node's four fields in one 16-byte access, addressed from the
+  // workgroup's first element so that the offsets stay small); otherwise it is field-major like
+  // the state. Measured: node-major took 2-4 % off the mixed stage at 5 and 8 nodes per line,
+  // and cost 4-7 % in FP64 (two half-line accesses per lane) and at 7 nodes per line.
+  constexpr bool kDuNodeMajor = std::is_same<DUT, float>::value && ES % 64 == 0;
+  Rsrc rp[4], rdu;
+  uint32_t du_off = 0;
   if constexpr (EPI == EPI_LSRK) {
+    if constexpr (kDuNodeMajor) {
+      const int ne = (E - e0 < EPB) ? E - e0 : EPB;
+      rdu = make_rsrc(static_cast<const DUT*>(L.du) + (size_t)e0 * ES * 4, (uint32_t)(ne * ES * 4 * sizeof(DUT)));
+      du_off = (uint32_t)((ge - e0) * ES + node) * (uint32_t)(4 * sizeof(DUT));
+    } else {
 #pragma unroll
-    for (int f = 0; f < 4; ++f) rp[f] = make_rsrc(static_cast<const DUT*>(L.du) + f * fs, fbdu);
+      for (int f = 0; f < 4; ++f) rp[f] = make_rsrc(static_cast<const DUT*>(L.du) + f * fs, (uint32_t)(fs * sizeof(DUT)));
+    }
   }
   auto load_pre = [&]() {
     if constexpr (EPI == EPI_LSRK) {
       if (L.a != 0.0) {
+        if constexpr (kDuNodeMajor) {
+          bload4f<kPolicyNT>(rdu, du_off, pre);
+        } else {
 #pragma unroll
-        for (int f = 0; f < 4; ++f) pre[f] = (double)bload<DUT, kPolicyNT>(rp[f], gi);
+          for (int f = 0; f < 4; ++f) pre[f] = (double)bload<DUT, kPolicyNT>(rp[f], gi);
+        }
       }
     }
     if constexpr (EPI == EPI_RK33) {
@@ -759,12 +796,14 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
       }
     } else {
       const float af = (float)ep_a, dtf = (float)ep_dt;
+      float dn[4];
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
-        const float dn = af * (float)pre[f] + dtf * (float)rhs[f];
-        bstore<float, kPolicyNT>(dn, rp[f], gi);
-        bstore<double, kPolicyNT>(q[f] + ep_b * (double)dn, make_rsrc(ep_qout + f * fs, fb64), gi);
+        dn[f] = af * (float)pre[f] + dtf * (float)rhs[f];
+        if constexpr (!kDuNodeMajor) bstore<float, kPolicyNT>(dn[f], rp[f], gi);
+        bstore<double, kPolicyNT>(q[f] + ep_b * (double)dn[f], make_rsrc(ep_qout + f * fs, fb64), gi);
       }
+      if constexpr (kDuNodeMajor) bstore4f<kPolicyNT>(dn, rdu, du_off);
     }
   } else {
     // Grid.cpp:705, 765-766, 826-827: out = (rhs*(dt*beta) + in*beta) + old*alpha
