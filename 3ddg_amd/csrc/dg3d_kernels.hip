@@ -137,6 +137,7 @@ __device__ __forceinline__ void bstore(T v, Rsrc r, uint32_t i) {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 // The four FP32 fields of one node of a node-major register at byte offset off, in one 16-byte
 // access (the whole vector is reinterpreted; no swizzle is bit-cast)
@@ -145,6 +146,20 @@ __device__ __forceinline__ void bload4f(Rsrc r, uint32_t off, double out[4]) {
   const f32x4 a = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, POL));
 #pragma unroll
   for (int k = 0; k < 4; ++k) out[k] = (double)a[k];
+}
+// Two FP64 fields of one node of a pair-major register (16 bytes)
+template <int POL = 0>
+__device__ __forceinline__ void bload2d(Rsrc r, uint32_t off, double& x0, double& x1) {
+  const f64x2 a = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, POL));
+  x0 = a[0];
+  x1 = a[1];
+}
+template <int POL = 0>
+__device__ __forceinline__ void bstore2d(double x0, double x1, Rsrc r, uint32_t off) {
+  f64x2 a;
+  a[0] = x0;
+  a[1] = x1;
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a), r, off, 0, POL);
 }
 template <int POL = 0>
 __device__ __forceinline__ void bstore4f(const float v[4], Rsrc r, uint32_t off) {
@@ -449,19 +464,28 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   double q[4], pre[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int f = 0; f < 4; ++f) q[f] = bload<double>(rq[f], gi);
-  // The LSRK register is never read across a face, so its layout is free. In FP32 with whole-wave
-  // elements it is node-major (a node's four fields in one 16-byte access, addressed from the
-  // workgroup's first element so that the offsets stay small); otherwise it is field-major like
-  // the state. Measured: node-major took 2-4 % off the mixed stage at 5 and 8 nodes per line,
-  // and cost 4-7 % in FP64 (two half-line accesses per lane) and at 7 nodes per line.
-  constexpr bool kDuNodeMajor = std::is_same<DUT, float>::value && ES % 64 == 0;
-  Rsrc rp[4], rdu;
+  // The LSRK register is never read across a face, so its layout is free. Where elements fill
+  // whole waves each lane moves its node's share in 16-byte accesses: the FP32 register is
+  // node-major (four fields in one access), the FP64 register pair-major (rho',u then v,w, two
+  // accesses into two arrays), each addressed from the workgroup's first element so that the
+  // offsets stay small. Otherwise it is field-major like the state.
+  constexpr bool kDu16 = ES % 64 == 0;
+  constexpr bool kDuNodeMajor = kDu16 && std::is_same<DUT, float>::value;
+  constexpr bool kDuPairMajor = kDu16 && std::is_same<DUT, double>::value;
+  Rsrc rp[4], rdu, rdu1;
   uint32_t du_off = 0;
   if constexpr (EPI == EPI_LSRK) {
-    if constexpr (kDuNodeMajor) {
+    if constexpr (kDu16) {
       const int ne = (E - e0 < EPB) ? E - e0 : EPB;
-      rdu = make_rsrc(static_cast<const DUT*>(L.du) + (size_t)e0 * ES * 4, (uint32_t)(ne * ES * 4 * sizeof(DUT)));
-      du_off = (uint32_t)((ge - e0) * ES + node) * (uint32_t)(4 * sizeof(DUT));
+      const uint32_t wbytes = (uint32_t)(ne * ES * 16);
+      const DUT* d0 = static_cast<const DUT*>(L.du);
+      if constexpr (kDuNodeMajor) {
+        rdu = make_rsrc(d0 + (size_t)e0 * ES * 4, wbytes);
+      } else {
+        rdu = make_rsrc(d0 + (size_t)e0 * ES * 2, wbytes);
+        rdu1 = make_rsrc(d0 + 2 * (size_t)fs + (size_t)e0 * ES * 2, wbytes);
+      }
+      du_off = (uint32_t)((ge - e0) * ES + node) * 16u;
     } else {
 #pragma unroll
       for (int f = 0; f < 4; ++f) rp[f] = make_rsrc(static_cast<const DUT*>(L.du) + f * fs, (uint32_t)(fs * sizeof(DUT)));
@@ -472,6 +496,9 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
       if (L.a != 0.0) {
         if constexpr (kDuNodeMajor) {
           bload4f<kPolicyNT>(rdu, du_off, pre);
+        } else if constexpr (kDuPairMajor) {
+          bload2d<kPolicyNT>(rdu, du_off, pre[0], pre[1]);
+          bload2d<kPolicyNT>(rdu1, du_off, pre[2], pre[3]);
         } else {
 #pragma unroll
           for (int f = 0; f < 4; ++f) pre[f] = (double)bload<DUT, kPolicyNT>(rp[f], gi);
@@ -788,11 +815,16 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     }
   } else if constexpr (EPI == EPI_LSRK) {
     if constexpr (std::is_same<DUT, double>::value) {
+      double dn[4];
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
-        const double dn = ep_a * pre[f] + ep_dt * (double)rhs[f];
-        bstore<double, kPolicyNT>(dn, rp[f], gi);
-        bstore<double, kPolicyNT>(q[f] + ep_b * dn, make_rsrc(ep_qout + f * fs, fb64), gi);
+        dn[f] = ep_a * pre[f] + ep_dt * (double)rhs[f];
+        if constexpr (!kDuPairMajor) bstore<double, kPolicyNT>(dn[f], rp[f], gi);
+        bstore<double, kPolicyNT>(q[f] + ep_b * dn[f], make_rsrc(ep_qout + f * fs, fb64), gi);
+      }
+      if constexpr (kDuPairMajor) {
+        bstore2d<kPolicyNT>(dn[0], dn[1], rdu, du_off);
+        bstore2d<kPolicyNT>(dn[2], dn[3], rdu1, du_off);
       }
     } else {
       const float af = (float)ep_a, dtf = (float)ep_dt;
