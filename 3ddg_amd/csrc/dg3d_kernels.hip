@@ -464,14 +464,15 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   double q[4], pre[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int f = 0; f < 4; ++f) q[f] = bload<double>(rq[f], gi);
-  // The LSRK register is never read across a face, so its layout is free. Where elements fill
-  // whole waves each lane moves its node's share in 16-byte accesses: the FP32 register is
-  // node-major (four fields in one access), the FP64 register pair-major (rho',u then v,w, two
-  // accesses into two arrays), each addressed from the workgroup's first element so that the
-  // offsets stay small. Otherwise it is field-major like the state.
-  constexpr bool kDu16 = ES % 64 == 0;
-  constexpr bool kDuNodeMajor = kDu16 && std::is_same<DUT, float>::value;
-  constexpr bool kDuPairMajor = kDu16 && std::is_same<DUT, double>::value;
+  // The LSRK register is never read across a face, so its layout is free. A lane can move its
+  // node's share in 16-byte accesses, addressed from the workgroup's first element so that the
+  // offsets stay small: the FP64 register pair-major (rho',u then v,w, two accesses into two
+  // arrays), the FP32 register node-major (four fields in one access). Otherwise it is
+  // field-major like the state. Measured per shape: the FP32 form pays only where elements fill
+  // whole waves, and at 7 nodes per line both forms ran 4-6 % slower than field-major
+  constexpr bool kDuNodeMajor = ES % 64 == 0 && std::is_same<DUT, float>::value;
+  constexpr bool kDuPairMajor = NN != 7 && std::is_same<DUT, double>::value;
+  constexpr bool kDu16 = kDuNodeMajor || kDuPairMajor;
   Rsrc rp[4], rdu, rdu1;
   uint32_t du_off = 0;
   if constexpr (EPI == EPI_LSRK) {
