@@ -468,10 +468,9 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   // node's share in 16-byte accesses, addressed from the workgroup's first element so that the
   // offsets stay small: the FP64 register pair-major (rho',u then v,w, two accesses into two
   // arrays), the FP32 register node-major (four fields in one access). Otherwise it is
-  // field-major like the state. Measured per shape: the FP32 form pays where elements fill whole
-  // waves and at 6 nodes per line, and at 7 nodes per line both forms ran 4-6 % slower than
-  // field-major
-  constexpr bool kDuNodeMajor = (ES % 64 == 0 || NN == 6) && std::is_same<DUT, float>::value;
+  // field-major like the state. Measured per shape: at 7 nodes per line both 16-byte forms ran
+  // 4-6 % slower than field-major; at every other order they are faster or even
+  constexpr bool kDuNodeMajor = NN != 7 && std::is_same<DUT, float>::value;
   constexpr bool kDuPairMajor = NN != 7 && std::is_same<DUT, double>::value;
   constexpr bool kDu16 = kDuNodeMajor || kDuPairMajor;
   Rsrc rp[4], rdu, rdu1;

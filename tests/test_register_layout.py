@@ -27,7 +27,7 @@ def layout(n, fp64):
     es = elem_stride(n)
     if fp64:
         return "pair" if n != 7 else "field"
-    return "node" if (es % 64 == 0 or n == 6) else "field"
+    return "node" if n != 7 else "field"
 
 
 def locations(n, epb, E, fp64):
