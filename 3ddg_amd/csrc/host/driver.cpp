@@ -29,6 +29,9 @@ const int ERRCODE_INVALID_ARGUMENT = 1;
 const int ERRCODE_INVALID_PARAMETER = 2;
 const int ERRCODE_CANNOT_READ = 3;
 const int ERRCODE_CANNOT_WRITE = 4;
+// Not in the reference: a HIP runtime or device failure, or an allocation failure, after the
+// inputs were read (the reference has no device and maps nothing to it)
+const int ERRCODE_DEVICE_FAILURE = 5;
 const char* NEEDED_ARGUMENTS =
     "	PARAMETER_FILENAME : path to the file defining simulation parameters\n"
     "	OUTPUT_FILENAME : path to the file that the data will be written to";
@@ -243,6 +246,11 @@ int main(int argc, char* argv[]) {
     Grid* newGrid = &g1;
     Grid* swapGrid = &g0;
     double t = 0.0;
+    // the reference's clock (driver.cpp:231,276): a float advanced by the float DT each step; it
+    // is the time printed on the progress lines and written to the HDF5 'time' datasets, while
+    // the solver keeps its own FP64 clock
+    float t_out = 0.0F;
+    const float DT_out = (float)DT;
     std::ofstream stepLog;  // RK33's log argument (unused there too)
     const auto begin = std::chrono::steady_clock::now();
     auto prev = begin;
@@ -264,7 +272,7 @@ int main(int argc, char* argv[]) {
           std::cout.width(3);
           std::cout << 100 * (i + j) / NUM_TIMESTEPS << "%) : t=";
           std::cout.width(5);
-          std::cout << t << "\n";
+          std::cout << t_out << "\n";
         }
         if (fused) {
           solver.check(dg3d_step(solver.ctx(), p.SCHEME == 1 ? DG3D_LSRK54 : DG3D_RK33, DT, 1, &t), "step");
@@ -276,18 +284,24 @@ int main(int argc, char* argv[]) {
           else newGrid->RK33(DT, oldGrid, &k1, &k2, stepLog);
           t += DT;
         }
+        t_out += DT_out;
       }
       if (p.BOOL_FILE) {
         solver.check(dg3d_get_state(solver.ctx(), fused ? 0 : newGrid->mBlock.reg(), st.data(), 4), "get_state");
-        out.dump_state(st, i + p.NUM_TIME_SUBSTEPS, t);
+        out.dump_state(st, i + p.NUM_TIME_SUBSTEPS, (double)t_out);
       }
     }
     std::cout << "Post-processing\n";
     const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - begin).count();
     std::cout << "Exiting\nTotal runtime is " << total << "s\n";
+  } catch (const std::bad_alloc& ex) {
+    std::cerr << "host allocation failed: " << ex.what() << "\n";
+    return ERRCODE_DEVICE_FAILURE;
   } catch (const std::exception& ex) {
+    // everything in this block runs after the inputs were read: a failure here is the device's
+    // (HIP error, device allocation) or the solver's, not an unreadable input
     std::cerr << ex.what() << "\n";
-    return ERRCODE_CANNOT_READ;
+    return ERRCODE_DEVICE_FAILURE;
   }
   return ERRCODE_SUCESSFUL_COMPLETION;
 }

@@ -2,17 +2,25 @@
 // loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the
 // product. Each function cites the reference lines it restates.
 //
-// Parity pins (tests/test_oracle_*.py, tests/test_basis.py):
+// Parity pins (tests/test_oracle_pins.py, tests/test_basis.py), each a closed form, an invariant,
+// brute force or the reference's own code, never this file's formulas retyped:
 //   LGL data and M, S: numpy.polynomial and closed forms;
 //   per-line solve: the reference's own Matrix.h QR solve (oracle/_ref);
-//   line operator: polynomial exactness; upwind flux: the interface transmission/reflection
-//     closed form; energy non-increase;
+//   line operator: polynomial exactness through the element loops (B/A = 0);
+//   kernel functions with B/A != 0 (kernelFuncs.cpp:13-30): exact derivatives of the quadratic
+//     equation of state and mass flux for linear rho', u, and the B/A == 0 switch;
+//   heterogeneous upwind flux (z1 != z2): a plane pulse crossing an impedance step reflects and
+//     transmits with R = (Z2 - Z1)/(Z1 + Z2), T = 2 Z2/(Z1 + Z2) (three pairs, both orientations);
+//   upwind flux at the zero-state halo: energy non-increase in an absorbing box;
+//   boundary source: a zero column driven by s(t) on the -z face carries s(t - z/c), with the
+//     error falling spectrally with the order;
 //   sponge sigma: tables by brute force; SSP-RK3 / LSRK54: orders 3 and 4 on y' = -sigma y;
 //   LDG (QFromDRho / DRhoFromQ): exactness for a cubic, heat-kernel decay of a Fourier mode;
 //   forwardEuler: order 1, one step (1 - sigma dt) exactly, equal to RK33's first stage, LDG
 //     on the x axis only;
 //   literal (FP32) mode: agreement with FP64 at FP32 level. Its bitwise FP32 rounding pattern
-//     is parity unpinned: the full reference needs MPI and HDF5 and is not built here.
+//     is parity unpinned: it would take a run of the whole reference (MPI, HDF5), which this
+//     checker does not build.
 #include "oracle.h"
 
 #include <algorithm>

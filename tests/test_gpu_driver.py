@@ -62,6 +62,11 @@ def _gather(p, out_dir, step):
         assert rho.shape == (8 * n, 8 * n, 16 * n) and vel.shape == (3, 8 * n, 8 * n, 16 * n)
         t = h5read.read(out_dir / "raw" / f"dRho_{rank}_{step}.h5", "time")
         assert t.shape == (1,)
+        # the reference's clock: float t += float DT once per step (driver.cpp:231,276)
+        tf = np.float32(0.0)
+        for _ in range(step):
+            tf = np.float32(tf + np.float32(0.000001))
+        assert t[0] == float(tf), (t[0], float(tf))
         for f, arr in enumerate([rho, vel[0], vel[1], vel[2]]):
             a = arr.reshape(8, n, 8, n, 16, n).transpose(0, 2, 4, 1, 3, 5)
             glob[f, bi * 8:(bi + 1) * 8, bj * 8:(bj + 1) * 8, :] = a

@@ -401,3 +401,62 @@ def test_fused_ldg_mixed_vs_fp64():
         ctx.close()
     err = rel_l2(st[1], st[0])
     assert 0.0 < err.max() < TOL_MIXED, err
+
+
+def _smooth_random_state(p, seed, rho_scale=0.3, u_scale=0.3):
+    """Random rho' (kg/m^3) and velocities (m/s) on every node: every flux term, including the
+    nonlinear rho'^2 and rho' u parts, carries data."""
+    rng = np.random.default_rng(seed)
+    st = np.empty((4, p.dof))
+    st[0] = rho_scale * (rng.random(p.dof) - 0.5)
+    st[1:] = u_scale * (rng.random((3, p.dof)) - 0.5)
+    return st.reshape(-1)
+
+
+@pytest.mark.parametrize("precision", [dg3d.FP64, dg3d.MIXED])
+@pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
+@pytest.mark.parametrize("order,n_e", [(7, 3), (1, 5)])
+def test_fused_orders_1_and_7_vs_oracle(order, n_e, scheme, precision):
+    """The fused step at 8 and 2 nodes per line, where the LSRK register has its own layouts
+    (field-major at 8 nodes, pair-major FP64 / node-major FP32 at 2) and the stage its own launch
+    bounds: heterogeneous c and rho0 with B/A = 5 everywhere, random state on every node, the tone
+    burst at its peak (the clock starts 30 steps before the envelope's centre), 60 steps. Every
+    variant of the stage kernel runs; 27 elements at N = 7 leave a partial last workgroup under
+    two elements per workgroup, 125 elements at N = 1 under both 32 and 16. FP64 equals the
+    oracle to 1e-12 and every FP64 variant gives the same bits; mixed is within 1e-5."""
+    print(f"fused_orders_1_7 order={order} n_e={n_e} scheme={scheme} precision={precision}", flush=True)
+    p = heterogeneous(small_c2(n_e=n_e, order=order, steps=60), seed=21)
+    p.b_over_a = np.full(p.E, 5.0)
+    st0 = _smooth_random_state(p, seed=22)
+    dt = problem.cfl_dt_host(p)
+    t0 = p.source.t0 - 30 * dt
+    ref = None
+    for v in range(8):
+        ctx = problem.make_context(p, precision=precision)
+        ctx.set_state(0, st0)
+        ctx.set_variant(scheme, v, 1)
+        t = ctx.step(scheme, dt, p.steps, t0)
+        st = ctx.get_state(0)
+        ctx.close()
+        assert abs(t - (t0 + p.steps * dt)) < 1e-12 * t
+        if ref is None:
+            ref = st
+        elif precision == dg3d.FP64:
+            assert np.array_equal(st, ref), v
+        else:
+            assert rel_l2(st, ref).max() < 1e-6, v
+    o = Oracle(p, literal=False, num_registers=4)
+    load_problem_into(o, p, 0)
+    o.set_state(0, st0)
+    o.set_time(t0)
+    for _ in range(p.steps):
+        if scheme == dg3d.RK33:
+            o.rk33(dt, 3, 0, 1, 2)
+            o.copy(0, 3)
+        else:
+            o.lsrk54(dt, 0, 1, 2)
+    r = o.get_state(0)
+    err = rel_l2(ref, r)
+    print(f"  rel L2 vs oracle {err}", flush=True)
+    assert np.linalg.norm(r - st0) > 1e-3 * np.linalg.norm(st0)
+    assert err.max() < (TOL_FP64 if precision == dg3d.FP64 else TOL_MIXED), err

@@ -419,3 +419,193 @@ def test_oracle_forward_euler_ldg_on_x_only(axis):
         assert np.abs(e_rho - k1_rho).max() < 1e-9 * change
     else:
         assert np.abs(e_rho - rho0).max() < 1e-9 * change
+
+
+# ---- closed forms of the heterogeneous flux, the nonlinear kernel functions and the source ----
+
+def _column(N, nz, length, c_lo, r_lo, c_hi=None, r_hi=None, k_if=None):
+    """A 1-D column along z: one element across x and y (periodic, so fields stay uniform in x and
+    y), nz elements of length/nz along z with zero-state (absorbing) ends. Elements k >= k_if take
+    the upper material."""
+    nE = (1, 1, nz)
+    widths = [np.array([length]), np.array([length]), np.full(nz, length / nz)]
+    p = problem._base("column", N, nE, (0, 0, 0), widths, c_lo, r_lo, 0.0)
+    p.periodic = (1, 1, 0)
+    if k_if is not None:
+        upper = np.arange(nz) >= k_if
+        p.c0 = np.where(upper, c_hi, c_lo)
+        p.rho0 = np.where(upper, r_hi, r_lo)
+    return p
+
+
+def _column_z(p):
+    """z coordinate of every node in the state layout [E][n^3] of a column."""
+    n = p.n
+    Z = problem.axis_coords(p, 2)                        # [nz][n]
+    return np.broadcast_to(Z[None, None, :, None, None, :], (1, 1, p.num_elements[2], n, n, n)).reshape(-1)
+
+
+def _run_lsrk54(o, dt, steps):
+    for _ in range(steps):
+        o.lsrk54(dt, 0, 1, 2, adv=1, diff=0, sponge=0)
+
+
+def _plane_wave_rt(z1_lo, z2_hi):
+    """A right-going Gaussian pressure pulse in the lower layer meets the interface at z = 1/2 and
+    splits. Returns the least-squares amplitudes of the reflected and transmitted pulses fitted
+    to the oracle's pressure and velocity, with the pulse shapes of the exact solution."""
+    (c1, r1), (c2, r2) = z1_lo, z2_hi
+    N, nz, L = 5, 32, 1.0
+    p = _column(N, nz, L, c1, r1, c2, r2, k_if=nz // 2)
+    Z1, Z2 = r1 * c1, r2 * c2
+    z = _column_z(p)
+    zi, z0, w, A = 0.5, 0.25, 0.05, 1e3
+    g = lambda s: np.exp(-((s - z0) / w) ** 2)
+    lower = np.broadcast_to(np.arange(nz)[:, None] < nz // 2, (nz, p.n ** 3)).reshape(-1)
+    st = np.zeros((4, z.size))
+    st[0] = np.where(lower, A * g(z) / c1 ** 2, 0.0)    # rho' = p / c^2
+    st[3] = np.where(lower, A * g(z) / Z1, 0.0)         # u_z = p / Z: right-going
+    o = ob.Oracle(p, num_registers=3)
+    o.set_state(0, st.reshape(-1))
+    for f in range(6):
+        if not p.periodic[problem.FACE_AXIS[f]]:
+            o.set_halo(0, f, None)
+    T = (zi - z0 + 0.25) / c1
+    dt0 = problem.cfl_dt_host(p)
+    steps = int(np.ceil(T / dt0))
+    _run_lsrk54(o, T / steps, steps)
+    out = o.get_state(0).reshape(4, -1)
+    c = np.where(lower, c1, c2)
+    Zn = np.where(lower, Z1, Z2)
+    pr = out[0] * c ** 2                                # linear: p = c^2 rho'
+    un = out[3]
+    # exact shapes at T: incident gone, reflected g(2 zi - z - c1 T), transmitted
+    # g(zi + (c1/c2)(z - zi) - c1 T); p_r = R A g_r, u_r = -p_r / Z1, p_t = T A g_t, u_t = p_t / Z2
+    gr = np.where(lower, g(2 * zi - z - c1 * T + 0.0), 0.0)
+    gt = np.where(lower, 0.0, g(zi + (c1 / c2) * (z - zi) - c1 * T))
+    gi = np.where(lower, g(z - c1 * T), 0.0)
+    assert gi.max() < 1e-9                              # the incident pulse has left the lower layer
+    # least squares on (p, Z u) together: p = A (R g_r + T g_t), Z u = A (-R g_r + T g_t)
+    M = np.stack([np.concatenate([gr, -gr]), np.concatenate([gt, gt])], axis=1) * A
+    rhs = np.concatenate([pr, Zn * un])
+    (R_fit, T_fit), *_ = np.linalg.lstsq(M, rhs, rcond=None)
+    resid = np.linalg.norm(M @ np.array([R_fit, T_fit]) - rhs) / np.linalg.norm(rhs)
+    return R_fit, T_fit, resid, (Z2 - Z1) / (Z1 + Z2), 2 * Z2 / (Z1 + Z2)
+
+
+@pytest.mark.parametrize("lo,hi", [((1500.0, 1000.0), (2000.0, 1500.0)),
+                                   ((2000.0, 1500.0), (1500.0, 1000.0)),
+                                   ((1500.0, 1000.0), (1540.0, 1050.0))])
+def test_oracle_interface_reflection_transmission(lo, hi):
+    """Heterogeneous upwind flux (Element.cpp:541-600, z1 != z2), B/A = 0: a plane pressure pulse
+    crossing an impedance step splits into a reflected pulse of amplitude R = (Z2 - Z1)/(Z1 + Z2)
+    and a transmitted one of amplitude T = 2 Z2/(Z1 + Z2), the transmitted pulse stretched by
+    c2/c1 (continuity of p and u at the interface). Fitted amplitudes agree to 1e-5 and the
+    fitted shapes explain the whole field to 1e-4."""
+    R_fit, T_fit, resid, R, T = _plane_wave_rt(lo, hi)
+    assert abs(R_fit - R) < 1e-5, (R_fit, R)
+    assert abs(T_fit - T) < 1e-5, (T_fit, T)
+    assert resid < 1e-4, resid
+
+
+def test_oracle_interface_closed_form_distinguishes_swap():
+    """The closed form separates the two orientations: with the layers swapped R changes sign
+    and T changes by the impedance ratio, far beyond the tolerance above."""
+    R_a, T_a, _, R, T = _plane_wave_rt((1500.0, 1000.0), (2000.0, 1500.0))
+    R_b, T_b, _, R2, T2 = _plane_wave_rt((2000.0, 1500.0), (1500.0, 1000.0))
+    assert abs(R + R2) < 1e-15 and abs(R) > 0.3
+    assert abs(R_a - R2) > 0.5 and abs(T_a - T2) > 0.5
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("ba", [5.0, 0.0])
+def test_oracle_nonlinear_kernel_functions_exact(axis, ba):
+    """kernelFuncs.cpp:13-30 through the element loops with B/A != 0: with rho' and u_axis linear
+    along the axis (and the exterior continuing them), the momentum mode gives
+    -d/dx[(rho' + rho'^2 (B/A)/(2 rho0)) c^2/rho0] and the mass mode -d/dx[(rho0 + rho') u]
+    exactly (both are quadratics, N = 3), from the second-order equation of state
+    p = c^2 rho' + (B/2A)(c^2/rho0) rho'^2 and the mass flux (rho0 + rho') u. At B/A == 0 exactly
+    the mass flux is rho0 u (the reference's switch, kernelFuncs.cpp:23,28): the rho' u term is
+    dropped although rho' is large."""
+    N = 3
+    c, r0 = 1500.0, 1000.0
+    p = _poly_problem(N)
+    p.b_over_a = np.full(p.E, ba)
+    o = ob.Oracle(p, num_registers=2)
+    coords = [problem.axis_coords(p, a) for a in range(3)]
+    n = p.n
+    nE = p.num_elements
+    X = [coords[0][:, None, None, :, None, None], coords[1][None, :, None, None, :, None],
+         coords[2][None, None, :, None, None, :]]
+    shape = tuple(nE) + (n, n, n)
+    xa = np.broadcast_to(X[axis], shape)
+    rho_l = np.polynomial.Polynomial([10.0, 40.0])       # rho' = 10 + 40 x (kg/m^3): 1-2 % of rho0
+    u_l = np.polynomial.Polynomial([0.5, -3.0])          # u_axis = 0.5 - 3 x (m/s)
+    st = np.zeros((4,) + shape)
+    st[0] = rho_l(xa)
+    st[1 + axis] = u_l(xa)
+    o.set_state(0, st.reshape(-1))
+    ecs = problem.element_corners(p.corner, p.widths)
+    for face in range(6):
+        a = problem.FACE_AXIS[face]
+        b_ax, c_ax = (1, 2) if a == 0 else ((0, 2) if a == 1 else (0, 1))
+        tr = np.zeros((nE[b_ax], nE[c_ax], n, n, 3))
+        if a == axis:
+            xf = (ecs[a][-1] + p.widths[a][-1]) if face % 2 else ecs[a][0]
+            tr[..., 0] = rho_l(xf)
+            tr[..., 1] = u_l(xf)
+        else:
+            cb = coords[axis]
+            arr = cb[:, None, :, None] if axis == b_ax else cb[None, :, None, :]
+            tr[..., 0] = rho_l(np.broadcast_to(arr, (nE[b_ax], nE[c_ax], n, n)))
+        o.set_halo(0, face, tr.reshape(-1))
+    eos = (rho_l + rho_l ** 2 * (ba / (2.0 * r0))) * (c * c / r0)      # P / rho0
+    mflux = (r0 + rho_l) * u_l if ba != 0.0 else r0 * u_l
+    for mode, fld, expect_poly in ((0, 1 + axis, -eos.deriv()), (1, 0, -mflux.deriv())):
+        o.zero(1)
+        o.operate(0, axis, mode, 0, 1)
+        got = o.get_state(1).reshape((4,) + shape)[fld]
+        expect = expect_poly(xa)
+        assert np.abs(got - expect).max() < 1e-10 * np.abs(expect).max(), (mode, np.abs(got - expect).max())
+    if ba != 0.0:
+        # the nonlinear parts are far above the tolerance: a dropped 1/2 or rho' u would fail
+        assert np.abs((rho_l ** 2 * (ba / (4.0 * r0)) * (c * c / r0)).deriv()(xa)).min() > 1e-3 * np.abs(eos.deriv()(xa)).max()
+        assert np.abs((rho_l * u_l).deriv()(xa)).max() > 1e-3 * np.abs(mflux.deriv()(xa)).max()
+
+
+@pytest.mark.parametrize("N,tol", [(4, 2e-3), (6, 5e-5)])
+def test_oracle_boundary_source_plane_wave(N, tol):
+    """Boundary source (no reference counterpart): a zero column, matched impedance, the tone
+    burst s(t) on the -z face. The upwind flux takes the incoming characteristic p + Z u = 2 s
+    from the exterior state (p, u_z) = (s, s/Z) and nothing from the zero interior, so the field
+    is the right-going wave p(z, t) = s(t - z/c), u_z = p/Z: the face node carries s(t) and the
+    error falls spectrally with the order (about 100x from N = 4 to N = 6). With the wrong sign of
+    the exterior u_z the incoming characteristic would be s - s = 0 and the column would stay at
+    rest; with a wrong 1/c^2 or 1/Z scale the amplitude would be off by orders of magnitude."""
+    c, r0 = 1500.0, 1000.0
+    nz, L = 40, 1.0
+    p = _column(N, nz, L, c, r0)
+    f = 15000.0
+    tau = 0.5 / f
+    t0 = 5 * tau                                         # envelope exp(-25) at t = 0: no front jump
+    amp, phase = 1e5, 0.7
+    o = ob.Oracle(p, num_registers=3)
+    o.zero(0)
+    o.set_source(0, amp, f, phase, t0, tau, None)
+    s = lambda t: amp * np.sin(2 * np.pi * f * t + phase) * np.exp(-((t - t0) / tau) ** 2)
+    T = 6.0 / f
+    dt0 = problem.cfl_dt_host(p)
+    steps = int(np.ceil(T / dt0))
+    o.set_time(0.0)
+    _run_lsrk54(o, T / steps, steps)
+    out = o.get_state(0).reshape(4, -1)
+    z = _column_z(p)
+    pe = s(T - z / c)
+    assert np.abs(pe).max() > 0.5 * amp                 # the burst is inside the column at T
+    err_p = np.abs(out[0] * c * c - pe).max() / amp
+    err_u = np.abs(out[3] * r0 * c - pe).max() / amp
+    print(f"N={N}: max error p {err_p:.2e}, Z u {err_u:.2e} (of the amplitude)")
+    assert err_p < tol and err_u < tol, (err_p, err_u)
+    face = z == 0.0
+    assert np.abs(out[0][face] * c * c - s(T)).max() / amp < tol
+    assert np.abs(out[1:3]).max() < 1e-12 * np.abs(out[3]).max()   # no transverse motion

@@ -2,8 +2,8 @@
 emulated on the CPU for every stage shape, both precisions and partial last workgroups.
 
 The register is never read across a face, so the kernel lays it out per precision and order:
-pair-major FP64 (two arrays of (rho', u) and (v, w) pairs) except at 7 nodes per line, node-major
-FP32 where elements fill whole waves or at 6 nodes per line, field-major otherwise. The 16-byte
+pair-major FP64 (two arrays of (rho', u) and (v, w) pairs) and node-major FP32 (the four fields of
+a node together) at every order except 7 nodes per line, where both stay field-major. The 16-byte
 forms are addressed from each workgroup's first element with a buffer resource of the
 workgroup's own size. The emulation checks that every (node slot, field) of every element gets
 its own location inside the 4 * fs values the context allocates, that each access stays inside
