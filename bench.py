@@ -50,6 +50,9 @@ def parse():
                     help="time one workload alone (for profiling): 64^3 N=7 or 48^3 N=5, mixed with the VALU or a "
                          "matrix-core contraction, or 48^3 N=5 FP64; prints one JSON line")
     ap.add_argument("--c5-steps", type=int, default=2000, help="steps of the combined 48^3 N=5 workload (0: skip)")
+    ap.add_argument("--precision-steps", type=int, default=500,
+                    help="least steps of the 64^3 N=4 and N=7 precision comparisons (they run on until the burst "
+                         "has passed the probe)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -183,20 +186,71 @@ def probe_node(p, depth_frac=0.05):
     return e * n ** 3 + (0 * n + 0) * n + 0
 
 
-def second_harmonic(trace, dt, f0):
-    """|DFT| of P(t) at 2 f0 over the recorded window (rectangular window), normalised."""
+def probe_depth(p, node):
+    """z coordinate of a probe node (global node index into a field)."""
+    n = p.n
+    e, nd = divmod(int(node), n ** 3)
+    k = e % p.num_elements[2]
+    zc = np.concatenate([[0.0], np.cumsum(p.widths[2])])
+    return float(zc[k] + p.widths[2][k] * (p.nodes[nd % n] / 2.0 + 0.5))
+
+
+def burst_window_steps(p, dt, depth):
+    """Steps until the whole tone burst (envelope e^-9 at t0 +- 3 tau) has passed a probe at
+    `depth`: the probe trace then holds the burst from before its start to after its end."""
+    s = p.source
+    return int(np.ceil((s.t0 + 3.0 * s.tau + depth / float(np.min(p.c0))) / dt))
+
+
+def hann(n):
+    return 0.5 * (1.0 - np.cos(2.0 * np.pi * np.arange(n) / max(n - 1, 1)))
+
+
+def dft_bin(trace, dt, freq):
+    """|sum w(t) x(t) e^(-2 pi i f t)| * 2 / sum w, Hann window w over the trace (the amplitude of
+    a steady sinusoid of that frequency)."""
+    w = hann(len(trace))
     t = np.arange(len(trace)) * dt
-    return float(np.abs(np.sum(trace * np.exp(-2j * np.pi * 2 * f0 * t))) * 2.0 / max(len(trace), 1))
+    return float(np.abs(np.sum(w * trace * np.exp(-2j * np.pi * freq * t))) * 2.0 / max(np.sum(w), 1e-300))
+
+
+def fubini_estimate(p, depth, dt, count):
+    """The plane-wave second harmonic expected at `depth` in the weak (pre-shock) regime, where the
+    Fubini solution reduces to its first term: p = p1(tau) + depth * beta/(2 rho0 c^3) *
+    d(p1^2)/dtau, with p1 the source burst delayed by depth/c and beta = 1 + B/(2A) (the
+    nonlinearity of this system's equation of state and mass flux, which also sets the Fubini
+    shock distance rho0 c^3/(beta omega p0)). Returns the 2 f0 and f0 bins of that waveform under
+    the same Hann window as the measured trace (so the burst's own spectral tail at 2 f0 enters
+    both), and sigma = depth / shock distance at the burst's peak."""
+    s = p.source
+    c, r0, ba = float(p.c0[0]), float(p.rho0[0]), float(p.b_over_a[0])
+    beta = 1.0 + ba / 2.0
+    t = (np.arange(count) + 1) * dt
+    tr = t - depth / c
+    p1 = s.amplitude * np.sin(2 * np.pi * s.frequency * tr + s.phase) * np.exp(-((tr - s.t0) / s.tau) ** 2)
+    dp1sq = np.gradient(p1 * p1, dt)
+    pq = p1 + depth * beta / (2.0 * r0 * c ** 3) * dp1sq
+    sigma = depth * beta * 2 * np.pi * s.frequency * s.amplitude / (r0 * c ** 3)
+    return dft_bin(pq, dt, 2 * s.frequency), dft_bin(pq, dt, s.frequency), sigma
 
 
 CONTRACTION_NAMES = ["valu", "mfma_f32", "mfma_bf16x3"]
 
 
-def compare_precisions(pk, p, steps, warmup, mixed_contraction=0, other_contractions=True):
-    """FP64 and mixed runs of the same problem: speed and accuracy of mixed at t_end (the mixed
-    run's derivative contraction: 0 VALU, 1 MFMA FP32, 2 MFMA split-BF16), and optionally the
-    speed and accuracy of the other two contractions."""
+def compare_precisions(pk, p, steps, warmup, mixed_contraction=0, other_contractions=True, timing_steps=20,
+                       until_burst_passed=False):
+    """FP64 and mixed runs of the same problem from t = 0: speed, and accuracy of mixed against
+    FP64 at t_end (the mixed run's derivative contraction: 0 VALU, 1 MFMA FP32, 2 MFMA
+    split-BF16). With until_burst_passed the runs go on until the whole tone burst has passed
+    the on-axis probe (at least `steps`), so that the probe trace holds it; the second harmonic
+    is measured there and set beside the plane-wave Fubini estimate. The other two contractions
+    are timed over `timing_steps` and compared with an FP64 run of the same length."""
+    import torch
     pr = probe_node(p)
+    depth = probe_depth(p, pr)
+    if until_burst_passed:
+        steps = max(steps, burst_window_steps(p, pk.problem.cfl_dt_host(p), depth) - warmup)
+    total = steps + warmup
     ctx64, r64 = timed_run(pk, p, pk.dg3d.FP64, steps, warmup, probe=pr)
     s64 = ctx64.get_state(0)
     tr64 = ctx64.get_probes()[:, 0]
@@ -207,31 +261,24 @@ def compare_precisions(pk, p, steps, warmup, mixed_contraction=0, other_contract
     ctx32.close()
     a = s32.reshape(4, -1)
     b = s64.reshape(4, -1)
+    del s32
     rel = [float(np.linalg.norm(a[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-300)) for f in range(4)]
-    c2 = (p.c0 ** 2)[:, None] * np.ones((1, p.n ** 3))
     ba = (p.b_over_a / (2 * p.rho0))[:, None]
-    P64 = (c2 * (b[0].reshape(p.E, -1) + ba * b[0].reshape(p.E, -1) ** 2))
-    P32 = (c2 * (a[0].reshape(p.E, -1) + ba * a[0].reshape(p.E, -1) ** 2))
-    pk64, pk32 = float(np.abs(P64).max()), float(np.abs(P32).max())
+    c2 = (p.c0 ** 2)[:, None]
+    r64p = b[0].reshape(p.E, -1)
+    r32p = a[0].reshape(p.E, -1)
+    pk64 = float(np.abs(c2 * (r64p + ba * r64p * r64p)).max())
+    pk32 = float(np.abs(c2 * (r32p + ba * r32p * r32p)).max())
+    del a, r32p
     f0 = p.source.frequency
-    h64 = second_harmonic(tr64, r64["dt"], f0)
-    h32 = second_harmonic(tr32, r32["dt"], f0)
+    dt = r64["dt"]
+    h64, h32 = dft_bin(tr64, dt, 2 * f0), dft_bin(tr32, dt, 2 * f0)
+    f64 = dft_bin(tr64, dt, f0)
     rate64 = p.dof * 5 * steps / r64["elapsed"]
     rate32 = p.dof * 5 * steps / r32["elapsed"]
-    others = {}
-    for con in range(len(CONTRACTION_NAMES)):
-        if not other_contractions or con == mixed_contraction:
-            continue
-        ctxm, rm = timed_run(pk, p, pk.dg3d.MIXED, steps, warmup, contraction=con)
-        sm = ctxm.get_state(0).reshape(4, -1)
-        ctxm.close()
-        relm = [float(np.linalg.norm(sm[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-300)) for f in range(4)]
-        others[CONTRACTION_NAMES[con]] = dict(stage_ms=rm["stage_ms"], variant=[rm["variant"], rm["xcd"]],
-                                              speedup_vs_fp64=r64["stage_ms"] / rm["stage_ms"],
-                                              rel_l2_max=max(relm))
-    return dict(
-        workload=p.name, elements=list(p.num_elements), order=p.order, dof=p.dof, steps=steps,
-        mixed_contraction=CONTRACTION_NAMES[mixed_contraction],
+    out = dict(
+        workload=p.name, elements=list(p.num_elements), order=p.order, dof=p.dof, steps=total, dt=dt,
+        t_end=r64["t_end"], timed_steps=steps, mixed_contraction=CONTRACTION_NAMES[mixed_contraction],
         fp64_dof_stage_per_s=rate64, mixed_dof_stage_per_s=rate32, mixed_speedup=rate32 / rate64,
         fp64_stage_ms=r64["stage_ms"], mixed_stage_ms=r32["stage_ms"],
         fp64_variant=[r64["variant"], r64["xcd"]], mixed_variant=[r32["variant"], r32["xcd"]],
@@ -239,9 +286,34 @@ def compare_precisions(pk, p, steps, warmup, mixed_contraction=0, other_contract
         mixed_hbm_frac=p.dof * BYTES_MIXED / (r32["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         mixed_rel_l2_max=max(rel), mixed_rel_l2=rel,
         peak_p_fp64=pk64, peak_p_rel_err=abs(pk32 - pk64) / max(pk64, 1e-300),
-        second_harmonic_fp64=h64, second_harmonic_rel_err=abs(h32 - h64) / max(h64, 1e-300),
-        other_contractions=others,
+        probe_depth_m=depth, probe_trace_max_p_fp64=float(np.abs(tr64).max()),
+        fundamental_fp64=f64, second_harmonic_fp64=h64, second_harmonic_ratio_fp64=h64 / max(f64, 1e-300),
+        second_harmonic_rel_err=abs(h32 - h64) / max(h64, 1e-300),
+        harmonic_window="Hann over the whole probe trace, t = dt .. t_end",
     )
+    if until_burst_passed and p.source.lead is None:
+        e2, e1, sigma = fubini_estimate(p, depth, dt, len(tr64))
+        out.update(fubini_second_harmonic=e2, fubini_ratio=e2 / max(e1, 1e-300), fubini_sigma=sigma,
+                   second_harmonic_vs_fubini=h64 / max(e2, 1e-300))
+    others = {}
+    if other_contractions:
+        short64, rs = timed_run(pk, p, pk.dg3d.FP64, timing_steps, warmup, variant=r64["variant"], xcd=r64["xcd"])
+        ref = short64.get_state(0).reshape(4, -1)
+        short64.close()
+        for con in range(len(CONTRACTION_NAMES)):
+            if con == mixed_contraction:
+                continue
+            ctxm, rm = timed_run(pk, p, pk.dg3d.MIXED, timing_steps, warmup, contraction=con)
+            sm = ctxm.get_state(0).reshape(4, -1)
+            ctxm.close()
+            relm = [float(np.linalg.norm(sm[f] - ref[f]) / max(np.linalg.norm(ref[f]), 1e-300)) for f in range(4)]
+            others[CONTRACTION_NAMES[con]] = dict(stage_ms=rm["stage_ms"], variant=[rm["variant"], rm["xcd"]],
+                                                  speedup_vs_fp64=r64["stage_ms"] / rm["stage_ms"],
+                                                  steps=timing_steps + warmup, rel_l2_max=max(relm))
+        del ref
+    out["other_contractions"] = others
+    torch.cuda.synchronize()
+    return out
 
 
 def batch_run(pk, steps, warmup=1):
@@ -310,29 +382,85 @@ def cpu_baseline(p, steps):
     return dict(value=p.dof * 5 * steps / el, unit="DOF-stage/s", cores=threads, kind="port",
                 sample=f"{steps} LSRK54 step(s) (5 stages) of the full {p.name} block "
                        f"({p.num_elements[0]}^3 elements, N={p.order}, {p.dof} nodes), FP64 oracle "
-                       f"with per-line dense solves, {el:.1f} s")
+                       f"with per-line dense solves, OpenMP over elements, {el:.1f} s",
+                host=host_cpu())
 
 
-def cpu_literal_baseline(p, steps):
-    """The reference's own arithmetic on the CPU: the oracle in literal mode (FP32 state and
-    flux constants, per-line dense solves, Grid::RK33 stages) on `steps` RK33 steps of the same
-    workload, OpenMP over elements on this host's cores (the reference runs one block per rank)."""
+def host_cpu():
+    """The host's CPU model (from /proc/cpuinfo) and the cores this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return dict(model=model, cpus_available=avail, cpus_online=os.cpu_count())
+
+
+def _split_blocks(nE, count):
+    """Blocks per axis (a product equal to `count`, each dividing nE) for one block per core:
+    the axis with the longest block is halved first."""
+    nb = [1, 1, 1]
+    left = count
+    while left > 1:
+        for f in (2, 3, 5, 7):
+            if left % f == 0:
+                break
+        else:
+            f = left
+        order = sorted(range(3), key=lambda a: -nE[a] // nb[a])
+        for a in order:
+            if (nE[a] // nb[a]) % f == 0:
+                nb[a] *= f
+                break
+        else:
+            break
+        left //= f
+    return nb
+
+
+def cpu_literal_baseline(pk, nE_full, order, h, cores, steps):
+    """The reference's own arithmetic and parallel model on the CPU: the block split into one
+    block per core (the reference's NUM_BLOCKS, one per single-threaded MPI rank), each block on
+    its own std::thread running the oracle in literal mode (FP32 state and flux constants,
+    per-line dense solves, Grid::RK33 stages). The blocks exchange no halos, so the MPI exchange
+    is left out of the time."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_bind import Oracle, load_problem_into
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    o = Oracle(p, literal=True, num_registers=4, threads=threads)
-    load_problem_into(o, p)
-    dt = importlib.import_module("3ddg_amd").problem.cfl_dt_host(p)
-    o.set_time(0.0)
+    import oracle_bind as ob
+    nb = _split_blocks(nE_full, cores)
+    sub = tuple(nE_full[a] // nb[a] for a in range(3))
+    blocks = []
+    for bi in range(nb[0]):
+        for bj in range(nb[1]):
+            for bk in range(nb[2]):
+                p = pk.problem._burst_problem("c2_block", order, sub, h, steps)
+                p.sigma = pk.dg3d.sponge_sigma(sub, tuple(nb), (bi, bj, bk), (h, h, h), (8, 8, 8, 8, 0, 8),
+                                               3 * 1500.0 / (8 * h), 1)
+                if bk > 0:
+                    p.source = pk.problem.Source()
+                o = ob.Oracle(p, literal=True, num_registers=4)
+                o.zero(0)
+                o.set_time(0.0)
+                blocks.append(o)
+    dt = pk.problem.cfl_dt_host(p)
     t0 = time.perf_counter()
-    for _ in range(steps):
-        o.rk33(dt, 3, 0, 1, 2)
-        o.copy(0, 3)
+    ob.rk33_blocks(blocks, dt, steps)
     el = time.perf_counter() - t0
-    del o
-    return dict(value=p.dof * 3 * steps / el, unit="DOF-stage/s", cores=threads, kind="port",
-                sample=f"{steps} RK33 step(s) (3 stages) of the full {p.name} block, literal FP32 oracle "
-                       f"(the reference's arithmetic), {el:.1f} s")
+    nodes = int(np.prod(nE_full)) * (order + 1) ** 3
+    del blocks
+    return dict(value=nodes * 3 * steps / el, unit="DOF-stage/s", cores=int(np.prod(nb)), kind="port",
+                sample=f"{steps} RK33 step(s) (3 stages) of the {nE_full[0]}x{nE_full[1]}x{nE_full[2]} N={order} "
+                       f"block split into {nb[0]}x{nb[1]}x{nb[2]} blocks of {sub[0]}x{sub[1]}x{sub[2]} elements, one "
+                       f"std::thread per block (the reference's one block per MPI rank, exchange left out), "
+                       f"literal FP32 oracle (the reference's arithmetic), {el:.1f} s",
+                host=host_cpu())
 
 
 def traffic_from_profiles(name):
@@ -433,11 +561,16 @@ def main():
                 stage_frac_of_ceiling=achieved / sc["stream_GBs"],
                 pattern=f"{sc['arrays_read']} FP64 arrays read and {sc['arrays_written']} written, nontemporal, "
                         f"256-lane workgroups streaming {sc['chunk']}-value chunks")
-        out["mixed_c2"] = compare_precisions(pk, pk.problem.config_c2(num_elements=args.elements, steps=args.steps),
-                                             args.steps, args.warmup)
+        # the precision comparisons run from t = 0 until the whole burst has passed the probe
+        # (at least the workloads' 500 steps), whatever --steps is
+        out["mixed_c2"] = compare_precisions(pk, pk.problem.config_c2(num_elements=args.elements),
+                                             args.precision_steps, args.warmup, timing_steps=args.steps,
+                                             until_burst_passed=True)
         # the N=7 precision workload runs its derivative contraction as split-BF16 MFMA GEMMs
-        out["mixed_c3"] = compare_precisions(pk, pk.problem.config_c3(num_elements=args.elements, steps=args.steps),
-                                             args.steps, args.warmup, mixed_contraction=pk.dg3d.CONTRACT_MFMA_BF16)
+        out["mixed_c3"] = compare_precisions(pk, pk.problem.config_c3(num_elements=args.elements),
+                                             args.precision_steps, args.warmup,
+                                             mixed_contraction=pk.dg3d.CONTRACT_MFMA_BF16, timing_steps=args.steps,
+                                             until_burst_passed=True)
     if not args.no_extra:
         out["small_c1"] = small_block_run(pk)
         out["other_paths_c2"] = other_paths_run(pk, args.elements, min(args.steps, 200), args.warmup)
@@ -446,12 +579,14 @@ def main():
     if not args.no_extra and args.c5_steps > 0:
         # combined workload: stretched two-layer 48^3 N=5 with a focused burst; mixed with the
         # split-BF16 matrix-core contraction against FP64
-        out["combined_c5"] = compare_precisions(pk, pk.problem.config_c5(steps=args.c5_steps), args.c5_steps,
-                                                args.warmup, mixed_contraction=pk.dg3d.CONTRACT_MFMA_BF16,
+        out["combined_c5"] = compare_precisions(pk, pk.problem.config_c5(steps=args.c5_steps),
+                                                args.c5_steps - args.warmup, args.warmup,
+                                                mixed_contraction=pk.dg3d.CONTRACT_MFMA_BF16,
                                                 other_contractions=False)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pk.problem.config_c2(num_elements=args.elements), args.cpu_steps)
-        out["cpu_literal_rk33"] = cpu_literal_baseline(pk.problem.config_c2(num_elements=args.elements),
+        cores = out["cpu_baseline"]["cores"]
+        out["cpu_literal_rk33"] = cpu_literal_baseline(pk, (args.elements,) * 3, 4, 0.75e-3, cores,
                                                        max(1, args.cpu_steps // 2))
     print(json.dumps(out))
 

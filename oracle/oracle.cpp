@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -751,6 +752,27 @@ void oracle_rk33(oracle* o, double dt, int nw, int old, int k1, int k2, int adv,
     oracle_axpby(o, nw, 1.0, old, 1.0 / 3.0);
   }
   o->time = t + dt;
+}
+
+void oracle_rk33_blocks(oracle* const* blocks, int count, double dt, int steps) {
+  // The reference's parallel model (driver.cpp:70-90, Grid.cpp:176-375): one block per MPI rank,
+  // each rank single-threaded. Here one std::thread per block, each running `steps` of
+  // Grid::RK33 on its block with registers 0 (state), 1, 2 (k1, k2) and 3 (new), then copying
+  // new into 0 as the driver's pointer swap does. The blocks exchange no halos: the timing
+  // leaves the reference's MPI exchange out.
+  std::vector<std::thread> pool;
+  pool.reserve((size_t)count);
+  for (int b = 0; b < count; ++b)
+    pool.emplace_back([=]() {
+#ifdef _OPENMP
+      omp_set_num_threads(1);  // this thread's own parallel regions run on itself only
+#endif
+      for (int s = 0; s < steps; ++s) {
+        oracle_rk33(blocks[b], dt, 3, 0, 1, 2, 1, 0, 1);
+        oracle_copy(blocks[b], 0, 3);
+      }
+    });
+  for (auto& t : pool) t.join();
 }
 
 void oracle_forward_euler(oracle* o, double dt, int nw, int old) {

@@ -75,6 +75,10 @@ void oracle_set_time(oracle* o, double t);
 /* Grid::RK33 (Grid.cpp:642-835): new_reg = RK33(old_reg) using k1, k2 registers. */
 void oracle_rk33(oracle* o, double dt, int new_reg, int old_reg, int k1, int k2, int eval_advect,
                  int eval_diff, int eval_sponge);
+/* `steps` of Grid::RK33 (advection and sponge, no diffusion) on each of `count` independent
+ * blocks, one std::thread per block with one OpenMP thread each: the reference's one block per
+ * single-threaded MPI rank, without the halo exchange. Registers 0..3 of every block are used. */
+void oracle_rk33_blocks(oracle* const* blocks, int count, double dt, int steps);
 /* Grid::forwardEuler (Grid.cpp:567-640): new_reg = old_reg + dt L(old_reg), with the LDG pair on
  * the x axis only, advection and sponge always on. Writes Q of old_reg. */
 void oracle_forward_euler(oracle* o, double dt, int new_reg, int old_reg);

@@ -59,6 +59,7 @@ def lib():
             "oracle_rk33": (None, [ctypes.c_void_p, ctypes.c_double] + [ctypes.c_int] * 7),
             "oracle_lsrk54": (None, [ctypes.c_void_p, ctypes.c_double] + [ctypes.c_int] * 6),
             "oracle_forward_euler": (None, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+            "oracle_rk33_blocks": (None, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_double, ctypes.c_int]),
             "oracle_maxabs": (None, [ctypes.c_void_p, ctypes.c_int, _dp]),
             "oracle_energy": (None, [ctypes.c_void_p, ctypes.c_int, _dp]),
             "oracle_sponge_sigma": (None, [_ip, _ip, _ip, _dp, _ip, ctypes.c_double, ctypes.c_int, _dp]),
@@ -184,6 +185,12 @@ class Oracle:
         out = np.zeros(2)
         self.L.oracle_energy(self.h, reg, _p(out))
         return out
+
+
+def rk33_blocks(oracles, dt, steps):
+    """`steps` RK33 steps on each oracle block, one std::thread per block (registers 0..3)."""
+    arr = (ctypes.c_void_p * len(oracles))(*[o.h for o in oracles])
+    lib().oracle_rk33_blocks(arr, len(oracles), float(dt), int(steps))
 
 
 def sponge_sigma(nE, nB, bpos, ew, sw, smax, ramp):
