@@ -180,7 +180,9 @@ __device__ __forceinline__ void st_stream(T v, T* p) { __builtin_nontemporal_sto
 //   F0 = K0 mom_o + K1 mom_s + K2 mass_o + K3 mass_s
 //   F1 = K4 mass_o + K5 mass_s + K6 mom_o + K7 mom_s
 // and the neighbour's kernel functions are mom_o = (rho + rho^2 K8) K9, mass_o = u (rho K11 + K10).
-constexpr int kNK = 12;
+// K12 = rho0_s / rho0_o converts a pressure shift expressed as P/rho0 of self into the
+// neighbour's P/rho0 (the FP32 stage's shifted lines, below).
+constexpr int kNK = 13;
 template <typename CT>
 __device__ __forceinline__ void face_coeffs(int side, CT c1, CT r1, CT ir1, CT c2, CT r2, CT ba2, CT ir2, CT* K) {
   const CT z1 = r1 * c1, z2 = r2 * c2;
@@ -264,18 +266,25 @@ __device__ __forceinline__ void column_line(int col, int& fo, int& oo, int& ls) 
 
 // One contraction column's line, mapped once per tile: loads and stores of its node m, which
 // read as 0 and are dropped past the line's end (m >= NN) and past the last column.
+// A P/rho0 column (field 0) reads its line shifted by the line's first value (see
+// kShiftPressureLines in stage_body); mass columns read the line as stored.
 template <int NN>
 struct TileLine {
   int fo, oo, ls;
   bool ok;
-  __device__ __forceinline__ TileLine(int col, int C) {
+  float sh;
+  __device__ __forceinline__ TileLine(int col, int C, const float* sF) {
     ok = col < C;
     column_line<NN>(ok ? col : 0, fo, oo, ls);  // branch-free: column 0 stands in past the end
+    const int rr = (ok ? col : 0) % (6 * NN * NN);
+    const bool pressure = ((rr / (NN * NN)) & 1) == 0;
+    const float v0 = sF[fo];
+    sh = pressure ? v0 : 0.f;
   }
   __device__ __forceinline__ float load(const float* sF, int m) const {
     const bool in = ok && m < NN;
     const float v = sF[in ? fo + m * ls : 0];
-    return in ? v : 0.f;
+    return in ? v - sh : 0.f;
   }
   __device__ __forceinline__ void store(float* sOut, int m, float v) const {
     if (ok && m < NN) sOut[oo + m * ls] = v;
@@ -283,7 +292,8 @@ struct TileLine {
 };
 
 // The workgroup's derivative contractions as one GEMM on the matrix cores:
-// out[i][col] = sum_m A[i][m] line_col[m], one column per (element, axis, field, face node).
+// out[i][col] = sum_m A[i][m] line_col[m], one column per (element, axis, field, face node);
+// P/rho0 lines enter shifted by their first value, as in the VALU contraction of the FP32 stage.
 // P copies of A sit block-diagonally in the 16x16 operand, block b on rows and K indices
 // [b*BS, b*BS + NN); the block stride BS (4 for NN <= 4, else 8) is a multiple of 4, so the 4
 // rows a lane holds (4g..4g+3, in both the K operand and the result) lie in one block. A tile
@@ -321,10 +331,10 @@ __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf,
       // K step s reads block (4s)/BS for every lane; the results go to block bd, whose line is
       // selected among the P mapped ones
       floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-      TileLine<NN> out(t * TC + j, C);
+      TileLine<NN> out(t * TC + j, C, sF);
 #pragma unroll
       for (int b = 0; b < P; ++b) {
-        const TileLine<NN> ln = b == 0 ? out : TileLine<NN>(t * TC + 16 * b + j, C);
+        const TileLine<NN> ln = b == 0 ? out : TileLine<NN>(t * TC + 16 * b + j, C, sF);
 #pragma unroll
         for (int s = b * BS / 4; s < (b + 1) * BS / 4; ++s)
           acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aop[s], ln.load(sF, 4 * s + g - b * BS), acc, 0, 0, 0);
@@ -333,6 +343,7 @@ __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf,
           out.oo = bd == b ? ln.oo : out.oo;
           out.ls = bd == b ? ln.ls : out.ls;
           out.ok = bd == b ? ln.ok : out.ok;
+          out.sh = bd == b ? ln.sh : out.sh;
         }
       }
 #pragma unroll
@@ -350,7 +361,7 @@ __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf,
       al[q] = l;
     }
     for (int t = wave; t < T; t += W) {
-      const TileLine<NN> ln(t * TC + 16 * bd + j, C);
+      const TileLine<NN> ln(t * TC + 16 * bd + j, C, sF);
       bf16x2v h0, m0, l0, h1, m1, l1;
       split3x2(float2v{ln.load(sF, md), ln.load(sF, md + 1)}, h0, m0, l0);
       split3x2(float2v{ln.load(sF, md + 2), ln.load(sF, md + 3)}, h1, m1, l1);
@@ -421,6 +432,16 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   constexpr int NP = S::NP, NF2 = S::NF2;
   static_assert(NT >= EPB * S::ES, "workgroup too small for the element group");
   static_assert(EPB <= 32 && NN * NN * NN <= 512, "face item encoding: local element in 5 bits, node in 9");
+  // FP32 stage: every line's P/rho0 values enter the contraction and the flux pair shifted by
+  // the line's first value c (node 0 along the axis), and the neighbour's by c rho0_s/rho0_o,
+  // i.e. the same pressure. A constant pressure drops out of the line's equation exactly: the
+  // upwind flux's pressure weights sum to one (z1 + z2 over z1 + z2), so it moves both face
+  // fluxes by the constant, which cancels the constant's volume term, and the jump terms see
+  // no change. In FP32 the shift removes the cancellation between a line's volume term and its
+  // lifted fluxes: on a line along which the pressure is (nearly) uniform, e.g. across a plane
+  // wave, the unshifted sum leaves rounding of the order of P/rho0 times the operator's norm
+  // in the transverse velocity, where the shifted one leaves (nearly) none. FP64 is unshifted.
+  constexpr bool kShiftPressureLines = !std::is_same<CT, double>::value;
   constexpr int NITEMS = EPB * 6 * NF2;            // face work items of the workgroup
   constexpr int R = (NITEMS + NT - 1) / NT;        // face items per lane
   CT* const sF = sm.sF;
@@ -677,6 +698,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     for (int k = 0; k < kNK; ++k) K[k] = CT(0);
     if (he < E) {
       face_coeffs<CT>(hdir & 1, (CT)ms_c, (CT)ms_r, (CT)ms_ir, (CT)mo_c, (CT)mo_r, (CT)mo_ba, (CT)mo_ir, K);
+      K[12] = (CT)(ms_r / mo_r);  // exactly 1 between elements of equal rho0
       if ((hdir & 1) == 0) sIw[hle * 3 + 2 - (hdir >> 1)] = (CT)hiw;
     }
 #pragma unroll
@@ -720,8 +742,18 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
       mass_o = Fo[(1 + axis) * NP + on];
     }
     CT* Fl = sFl + ((fle * 6 + dir) * NF2 + fn) * 2;
-    Fl[0] = K[0] * mom_o + K[1] * mom_s + K[2] * mass_o + K[3] * mass_s;
-    Fl[1] = K[4] * mass_o + K[5] * mass_s + K[6] * mom_o + K[7] * mom_s;
+    if constexpr (kShiftPressureLines) {
+      // the shift of the line through this face node along the face's axis: its node 0
+      const int b = fn / NN, cc = fn - (fn / NN) * NN;
+      const CT c0 = (dir & 1) ? F[face_node<NN>(axis, 0, b, cc)] : mom_s;
+      mom_o -= c0 * K[12];
+      const CT mom_sd = mom_s - c0;
+      Fl[0] = K[0] * mom_o + K[1] * mom_sd + K[2] * mass_o + K[3] * mass_s;
+      Fl[1] = K[4] * mass_o + K[5] * mass_s + K[6] * mom_o + K[7] * mom_sd;
+    } else {
+      Fl[0] = K[0] * mom_o + K[1] * mom_s + K[2] * mass_o + K[3] * mass_s;
+      Fl[1] = K[4] * mass_o + K[5] * mass_s + K[6] * mom_o + K[7] * mom_s;
+    }
     if constexpr (LDG) {
       double vq = nq[r];
       if constexpr (!std::is_same<CT, double>::value) asm volatile("" : "+v"(vq));
@@ -763,10 +795,12 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
       dp = sOut[(le * 6 + 2 * a) * NP + node];
       dm = sOut[(le * 6 + 2 * a + 1) * NP + node];
     } else {
+      const CT c0 = kShiftPressureLines ? F[base] : CT(0);
 #pragma unroll
       for (int mm = 0; mm < NN; ++mm) {
         const CT av = sA[ix * NN + mm];
-        dp += av * F[base + mm * ls];
+        if constexpr (kShiftPressureLines) dp += av * (F[base + mm * ls] - c0);
+        else dp += av * F[base + mm * ls];
         dm += av * Fa[base + mm * ls];
       }
     }
@@ -1587,6 +1621,7 @@ hipError_t launch_cfl(const MeshArgs& m, double dxi_half, double* part, hipStrea
   hipLaunchKernelGGL(k_cfl, dim3(reduce_blocks(m.E)), dim3(kRedThreads), 0, s, m, dxi_half, part);
   return hipGetLastError();
 }
+
 hipError_t launch_probe(const double* reg, const MeshArgs& m, int n, const int64_t* nodes, int count, double* out,
                         const int* step, int cap, hipStream_t s) {
   if (count <= 0) return hipSuccess;

@@ -6,7 +6,8 @@ aligned diagonal blocks (one every 4 rows for n <= 4, every 8 rows for n >= 5), 
 v_mfma_f32_16x16x4f32 and v_mfma_f32_16x16x16bf16_1k, and the round-robin of tiles over waves.
 It checks what the kernel relies on without a bounds check of its own: every LDS read and write
 stays inside its array, every output line is written exactly once, and the tiled GEMM equals
-the direct per-node contraction out[i] = sum_m A[i][m] line[m] along each axis.
+the direct per-node contraction out[i] = sum_m A[i][m] line[m] along each axis, where a P/rho0
+line enters shifted by its first value (the FP32 stage's shifted pressure lines).
 """
 import numpy as np
 import pytest
@@ -47,15 +48,18 @@ def emulate(n, epb, mode, A, sF):
 
     def line(col):
         ok = col < C
-        fo, oo, ls = column_line(n, col if ok else 0)
-        return ok, fo, oo, ls
+        c = col if ok else 0
+        fo, oo, ls = column_line(n, c)
+        assert 0 <= fo < sF.size
+        pressure = ((c % (6 * NF2)) // NF2) % 2 == 0
+        return ok, fo, oo, ls, (sF[fo] if pressure else 0.0)
 
     def load(ln, m):
-        ok, fo, _, ls = ln
+        ok, fo, _, ls, sh = ln
         if ok and m < n:
             i = fo + m * ls
             assert 0 <= i < sF.size
-            return sF[i]
+            return sF[i] - sh
         return 0.0
 
     for wave in range(W):
@@ -87,7 +91,7 @@ def emulate(n, epb, mode, A, sF):
                     out.append((j, [(4 * g + r, lines[bd], md + r) for r in range(4)]))
             D = Aop @ Bop
             for j, rows in out:
-                for row, (ok, _, oo, ls), m in rows:
+                for row, (ok, _, oo, ls, _sh), m in rows:
                     if ok and m < n:
                         i = oo + m * ls
                         assert 0 <= i < sOut.size
@@ -106,7 +110,8 @@ def direct(n, epb, A, sF):
                 ls = NF2 if a == 0 else (n if a == 1 else 1)
                 base = node - idx[a] * ls
                 F = le * 4 * NP
-                ref[(le * 6 + 2 * a) * NP + node] = sum(A[idx[a], m] * sF[F + base + m * ls] for m in range(n))
+                ref[(le * 6 + 2 * a) * NP + node] = sum(A[idx[a], m] * (sF[F + base + m * ls] - sF[F + base])
+                                                        for m in range(n))
                 ref[(le * 6 + 2 * a + 1) * NP + node] = sum(
                     A[idx[a], m] * sF[F + (1 + a) * NP + base + m * ls] for m in range(n))
     return ref
