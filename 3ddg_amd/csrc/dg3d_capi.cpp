@@ -87,6 +87,11 @@ struct dg3d_ctx {
   int probe_count = 0, probe_cap = 0, probe_steps = 0;
   double* d_part = nullptr;
   std::vector<double> h_part;
+  // the allocation-free reductions: this device's copy of the library's scratch array, resolved
+  // when the context is created (null, with the error kept, if that failed)
+  int device = 0;
+  double* scratch = nullptr;
+  hipError_t scratch_err = hipSuccess;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int tune_variant[2] = {-1, -1};  // per scheme: chosen kernel variant (-1 = not tuned yet)
   int tune_xcd[2] = {1, 1};
@@ -96,6 +101,13 @@ struct dg3d_ctx {
 
   int fail(int code, const std::string& msg) {
     err = msg;
+    return code;
+  }
+  // a message written into err's reserved capacity, without a temporary string (the
+  // allocation-free entry points report through this)
+  int fail_text(int code, const char* a, const char* b = "") {
+    err.assign(a);
+    err.append(b);
     return code;
   }
   int hip_fail(hipError_t e, const char* what) {
@@ -545,7 +557,14 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
       free_ctx(c);
       return DG3D_E_INVALID;
     }
-    hipError_t e = hipStreamCreate(&c->stream);
+    c->err.reserve(256);
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipStreamCreate(&c->stream);
+    if (e == hipSuccess) {
+      // once per device, with the context's device current: the scratch array's address
+      c->scratch_err = dg3d::scratch_address(c->device, &c->scratch);
+      if (c->scratch_err != hipSuccess) c->scratch = nullptr;
+    }
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e != hipSuccess) {
@@ -932,61 +951,106 @@ int dg3d_cfl_dt(dg3d_ctx* c, double cfl, double* dt) {
   return DG3D_OK;
 }
 
-int dg3d_reduce(dg3d_ctx* c, int kind, int reg, int reg2, double* out) {
-  if (!c || !c->reg_ok(reg) || !out) return c ? c->fail(DG3D_E_INVALID, "reduce: bad register") : DG3D_E_INVALID;
-  hipError_t e;
-  const int blocks = dg3d::reduce_blocks(c->fs);
+namespace {
+
+// Per-block values a reduction kind writes, and the combine kind of dg3d::launch_reduce_combine.
+bool reduce_shape(int kind, int* per_block, int* combine) {
+  if (kind == DG3D_REDUCE_MAXABS) { *per_block = 4; *combine = 0; return true; }
+  if (kind == DG3D_REDUCE_ENERGY) { *per_block = 2; *combine = 1; return true; }
+  if (kind == DG3D_REDUCE_RELL2 || kind == DG3D_REDUCE_DIFFL2) { *per_block = 8; *combine = 2; return true; }
+  return false;
+}
+
+hipError_t launch_reduction(dg3d_ctx* c, int kind, int reg, int reg2, double* part) {
+  if (kind == DG3D_REDUCE_MAXABS) return dg3d::launch_maxabs(c->regs[reg].d, c->fs, 4, part, c->stream);
+  if (kind == DG3D_REDUCE_ENERGY) return dg3d::launch_energy(c->regs[reg].d, c->mesh(), c->n, c->d_wq, part, c->stream);
+  return dg3d::launch_l2diff(c->regs[reg].d, c->regs[reg2].d, c->fs, 4, part, c->stream);
+}
+
+// The outputs of a reduction kind from its combined values (v: 4 maxima, the 2 energy sums, or
+// the 4 squared-difference sums then the 4 squared-reference sums).
+int finish_reduce(dg3d_ctx* c, int kind, const double* v, double* out) {
   if (kind == DG3D_REDUCE_MAXABS) {
-    e = dg3d::launch_maxabs(c->regs[reg].d, c->fs, 4, c->d_part, c->stream);
-    if (e != hipSuccess) return c->hip_fail(e, "reduce");
-    int rc = reduce_partials(c, 4, blocks);
-    if (rc) return rc;
-    for (int f = 0; f < 4; ++f) {
-      double m = 0.0;
-      for (int b = 0; b < blocks; ++b) {
-        const double v = c->h_part[b * 4 + f];
-        if (std::isnan(v) || std::isnan(m)) m = std::nan("");
-        else m = std::max(m, v);
-      }
-      out[f] = m;
-    }
+    for (int f = 0; f < 4; ++f) out[f] = v[f];
     return DG3D_OK;
   }
   if (kind == DG3D_REDUCE_ENERGY) {
-    e = dg3d::launch_energy(c->regs[reg].d, c->mesh(), c->n, c->d_wq, c->d_part, c->stream);
-    if (e != hipSuccess) return c->hip_fail(e, "reduce");
-    int rc = reduce_partials(c, 2, blocks);
-    if (rc) return rc;
-    out[0] = out[1] = 0.0;
-    for (int b = 0; b < blocks; ++b) {
-      out[0] += c->h_part[b * 2];
-      out[1] += c->h_part[b * 2 + 1];
-    }
-    if (!std::isfinite(out[0])) return c->fail(DG3D_E_NAN, "energy is not finite");
+    out[0] = v[0];
+    out[1] = v[1];
+    if (!std::isfinite(out[0])) return c->fail_text(DG3D_E_NAN, "energy is not finite");
     return DG3D_OK;
   }
-  if (kind == DG3D_REDUCE_RELL2 || kind == DG3D_REDUCE_DIFFL2) {
-    if (!c->reg_ok(reg2)) return c->fail(DG3D_E_INVALID, "reduce: bad second register");
-    e = dg3d::launch_l2diff(c->regs[reg].d, c->regs[reg2].d, c->fs, 4, c->d_part, c->stream);
-    if (e != hipSuccess) return c->hip_fail(e, "reduce");
-    int rc = reduce_partials(c, 8, blocks);
-    if (rc) return rc;
-    double d[4] = {0, 0, 0, 0}, r[4] = {0, 0, 0, 0};
+  for (int f = 0; f < 4; ++f) {
+    if (kind == DG3D_REDUCE_RELL2) out[f] = (v[4 + f] > 0.0) ? std::sqrt(v[f] / v[4 + f]) : std::sqrt(v[f]);
+    else {
+      out[f] = std::sqrt(v[f]);
+      out[4 + f] = std::sqrt(v[4 + f]);
+    }
+  }
+  return DG3D_OK;
+}
+
+}  // namespace
+
+int dg3d_reduce(dg3d_ctx* c, int kind, int reg, int reg2, double* out) {
+  if (!c || !c->reg_ok(reg) || !out) return c ? c->fail(DG3D_E_INVALID, "reduce: bad register") : DG3D_E_INVALID;
+  int per_block = 0, combine = 0;
+  if (!reduce_shape(kind, &per_block, &combine)) return c->fail(DG3D_E_INVALID, "reduce: unknown kind");
+  if (combine == 2 && !c->reg_ok(reg2)) return c->fail(DG3D_E_INVALID, "reduce: bad second register");
+  const int blocks = dg3d::reduce_blocks(c->fs);
+  hipError_t e = launch_reduction(c, kind, reg, reg2, c->d_part);
+  if (e != hipSuccess) return c->hip_fail(e, "reduce");
+  int rc = reduce_partials(c, per_block, blocks);
+  if (rc) return rc;
+  // combined on the host in block order (dg3d::launch_reduce_combine combines in the same order)
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double* h = c->h_part.data();
+  if (combine == 0) {
+    for (int f = 0; f < 4; ++f) {
+      double m = 0.0;
+      for (int b = 0; b < blocks; ++b) {
+        const double x = h[b * 4 + f];
+        if (std::isnan(x) || std::isnan(m)) m = std::nan("");
+        else m = std::max(m, x);
+      }
+      v[f] = m;
+    }
+  } else if (combine == 1) {
+    for (int b = 0; b < blocks; ++b) {
+      v[0] += h[b * 2];
+      v[1] += h[b * 2 + 1];
+    }
+  } else {
     for (int b = 0; b < blocks; ++b)
       for (int f = 0; f < 4; ++f) {
-        d[f] += c->h_part[(b * 4 + f) * 2];
-        r[f] += c->h_part[(b * 4 + f) * 2 + 1];
+        v[f] += h[(b * 4 + f) * 2];
+        v[4 + f] += h[(b * 4 + f) * 2 + 1];
       }
-    for (int f = 0; f < 4; ++f) {
-      if (kind == DG3D_REDUCE_RELL2) out[f] = (r[f] > 0.0) ? std::sqrt(d[f] / r[f]) : std::sqrt(d[f]);
-      else {
-        out[f] = std::sqrt(d[f]);
-        out[4 + f] = std::sqrt(r[f]);
-      }
-    }
-    return DG3D_OK;
   }
-  return c->fail(DG3D_E_INVALID, "reduce: unknown kind");
+  return finish_reduce(c, kind, v, out);
+}
+
+int dg3d_reduce_noalloc(dg3d_ctx* c, int kind, int reg, int reg2, double* out) {
+  if (!c) return DG3D_E_INVALID;
+  if (!c->reg_ok(reg) || !out) return c->fail_text(DG3D_E_INVALID, "reduce_noalloc: bad register or output");
+  int per_block = 0, combine = 0;
+  if (!reduce_shape(kind, &per_block, &combine)) return c->fail_text(DG3D_E_INVALID, "reduce_noalloc: unknown kind");
+  if (combine == 2 && !c->reg_ok(reg2)) return c->fail_text(DG3D_E_INVALID, "reduce_noalloc: bad second register");
+  if (!c->scratch)
+    return c->fail_text(DG3D_E_HIP, "reduce_noalloc: the device scratch could not be resolved: ",
+                        hipGetErrorString(c->scratch_err));
+  const int blocks = dg3d::reduce_blocks(c->fs);
+  if ((int64_t)blocks * per_block > dg3d::kScratchPartials)
+    return c->fail_text(DG3D_E_INVALID, "reduce_noalloc: the reduction needs more partials than the scratch holds");
+  double* part = c->scratch;
+  double* vals = c->scratch + dg3d::kScratchPartials;
+  hipError_t e = launch_reduction(c, kind, reg, reg2, part);
+  if (e == hipSuccess) e = dg3d::launch_reduce_combine(part, blocks, combine, vals, c->stream);
+  double v[dg3d::kScratchValues];
+  if (e == hipSuccess) e = hipMemcpyAsync(v, vals, sizeof(v), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->fail_text(DG3D_E_HIP, "reduce_noalloc: ", hipGetErrorString(e));
+  return finish_reduce(c, kind, v, out);
 }
 
 int dg3d_set_probes(dg3d_ctx* c, int count, const int64_t* nodes, int capacity_steps) {

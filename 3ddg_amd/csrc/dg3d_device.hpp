@@ -276,6 +276,17 @@ hipError_t launch_l2diff(const double* a, const double* b, int64_t fs, int nfiel
 hipError_t launch_energy(const double* reg, const MeshArgs& m, int n, const double* wq, double* partials,
                          hipStream_t s);
 hipError_t launch_cfl(const MeshArgs& m, double dxi_half, double* partials, hipStream_t s);
+
+// The allocation-free reductions' scratch: a file-scope device array of kScratchPartials
+// per-block partials (8 per block at the most blocks reduce_blocks gives) and kScratchValues
+// combined values, one copy per device. scratch_address resolves its device address for
+// `device`, which must be the current device, once per device (later calls return the kept
+// address). launch_reduce_combine combines `blocks` partials of a reduction kind (0 max |.| of
+// 4 fields, 1 energy sums, 2 L2 sums of 4 fields) in block order into out[0..7].
+constexpr int kScratchPartials = 1024 * 8;
+constexpr int kScratchValues = 8;
+hipError_t scratch_address(int device, double** out);
+hipError_t launch_reduce_combine(const double* partials, int blocks, int kind, double* out, hipStream_t s);
 // Probe values into row *step of `out` ([cap][count]); nothing once *step reaches cap.
 hipError_t launch_probe(const double* reg, const MeshArgs& m, int n, const int64_t* nodes, int count,
                         double* out, const int* step, int cap, hipStream_t s);

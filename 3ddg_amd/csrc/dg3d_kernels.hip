@@ -13,6 +13,7 @@
 //      fluxes with M^{-1} e_0 / M^{-1} e_N (the algebraic form of Element.cpp:602), applies
 //      the element sponge (Element.cpp:760-781) and the RK update in registers.
 // The same kernel, with a term mask and an accumulate epilogue, implements Block::Operate*.
+#include <mutex>
 #include <type_traits>
 
 #include "dg3d_device.hpp"
@@ -1622,6 +1623,74 @@ hipError_t launch_cfl(const MeshArgs& m, double dxi_half, double* part, hipStrea
   return hipGetLastError();
 }
 
+// ---- the allocation-free reductions ---------------------------------------------------------
+// Per-block partials of a reduction and its combined values live in this array, one copy per
+// device (each device holds its own until the process ends). Host code never passes the array's
+// name to a kernel: it resolves the device address once per device (scratch_address) and passes
+// that. Every reduction launch writes every partial the combine then reads, and the combine
+// writes every value the host then reads, so nothing has to start at zero; calls that overlap on
+// one device race on the array.
+static_assert(kScratchPartials >= kRedMaxBlocks * 8, "8 partials per block at the most blocks");
+static __device__ __align__(256) double scratch[kScratchPartials + kScratchValues];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+hipError_t scratch_address(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[device]) {
+    // the address is resolved with `device` current, as the caller's context was created on it
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch[device];
+  return hipSuccess;
+}
+
+// One lane combines the partials in block order, the order dg3d_reduce sums them in on the
+// host: NaN-sticky max of 4 fields, or sums of per_block values.
+__global__ void k_reduce_combine(const double* part, int blocks, int kind, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (kind == 0) {
+    for (int f = 0; f < 4; ++f) {
+      double m = 0.0;
+      for (int b = 0; b < blocks; ++b) {
+        const double v = part[b * 4 + f];
+        m = (isnan(v) || isnan(m)) ? __builtin_nan("") : (m > v ? m : v);
+      }
+      out[f] = m;
+    }
+  } else if (kind == 1) {
+    double e = 0.0, r = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+      e += part[b * 2];
+      r += part[b * 2 + 1];
+    }
+    out[0] = e;
+    out[1] = r;
+  } else {
+    for (int f = 0; f < 4; ++f) {
+      double d = 0.0, r = 0.0;
+      for (int b = 0; b < blocks; ++b) {
+        d += part[(b * 4 + f) * 2];
+        r += part[(b * 4 + f) * 2 + 1];
+      }
+      out[f] = d;
+      out[4 + f] = r;
+    }
+  }
+}
+
+hipError_t launch_reduce_combine(const double* part, int blocks, int kind, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_combine, dim3(1), dim3(64), 0, s, part, blocks, kind, out);
+  return hipGetLastError();
+}
 hipError_t launch_probe(const double* reg, const MeshArgs& m, int n, const int64_t* nodes, int count, double* out,
                         const int* step, int cap, hipStream_t s) {
   if (count <= 0) return hipSuccess;

@@ -106,6 +106,7 @@ _SIGS = {
     "dg3d_group_order": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int64,
                                         ctypes.POINTER(ctypes.c_int64)]),
     "dg3d_reduce": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
+    "dg3d_reduce_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "dg3d_set_probes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
                                         ctypes.c_int]),
     "dg3d_get_probes": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip]),
@@ -353,6 +354,12 @@ class Context:
     def reduce(self, kind, reg, reg2=0):
         out = np.zeros(8)
         self._check(self._L.dg3d_reduce(self._h, kind, reg, reg2, _ptr(out)), "reduce")
+        return out
+
+    def reduce_noalloc(self, kind, reg, reg2=0):
+        """dg3d_reduce_noalloc: the same reduction with the library's per-device scratch array."""
+        out = np.zeros(8)
+        self._check(self._L.dg3d_reduce_noalloc(self._h, kind, reg, reg2, _ptr(out)), "reduce_noalloc")
         return out
 
     def set_probes(self, nodes, capacity):

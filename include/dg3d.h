@@ -190,6 +190,18 @@ int dg3d_cfl_dt(dg3d_ctx* ctx, double cfl, double* dt);
 /* Reductions over a register (see DG3D_REDUCE_*); out holds up to 8 doubles. */
 int dg3d_reduce(dg3d_ctx* ctx, int kind, int reg, int reg2, double* out);
 
+/* Allocation-free variant of dg3d_reduce. No counterpart in the reference: it is for callers
+ * that may not allocate (serving loops, harnesses that forbid allocation). Same arguments and
+ * results, bit for bit, as dg3d_reduce. The per-block partials and combined values live in a
+ * file-scope device array of the library, 8200 doubles (64.1 KiB), one copy per device, held
+ * until the process ends; its address is resolved when a context is created on that device. The
+ * call allocates no device or host memory; its error messages go into space the context
+ * reserved. A grid-stride reduction of at most 1024 blocks needs at most 8192 partials, so every
+ * register fits; a reduction needing more is rejected with DG3D_E_INVALID. Calls that overlap on
+ * one device (contexts on one GPU used from several host threads) race on the array; calls on
+ * one context are ordered by its stream. Synchronises, as dg3d_reduce does. */
+int dg3d_reduce_noalloc(dg3d_ctx* ctx, int kind, int reg, int reg2, double* out);
+
 /* Probe nodes: after every step of dg3d_step the pressure P(rho') at each probe node
  * (global node index into a field) is recorded. */
 int dg3d_set_probes(dg3d_ctx* ctx, int count, const int64_t* node_index, int capacity_steps);
