@@ -237,14 +237,30 @@ def fubini_estimate(p, depth, dt, count):
 CONTRACTION_NAMES = ["valu", "mfma_f32", "mfma_bf16x3"]
 
 
+def _mixed_vs(p, a, b, tr_a, tr_b, dt, f0):
+    """Accuracy of a mixed state `a` (and probe trace) against the FP64 state `b`: rel L2 per
+    field, peak pressure and second-harmonic errors."""
+    rel = [float(np.linalg.norm(a[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-300)) for f in range(4)]
+    ba = (p.b_over_a / (2 * p.rho0))[:, None]
+    c2 = (p.c0 ** 2)[:, None]
+    ra = a[0].reshape(p.E, -1)
+    rb = b[0].reshape(p.E, -1)
+    pka = float(np.abs(c2 * (ra + ba * ra * ra)).max())
+    pkb = float(np.abs(c2 * (rb + ba * rb * rb)).max())
+    ha, hb = dft_bin(tr_a, dt, 2 * f0), dft_bin(tr_b, dt, 2 * f0)
+    return dict(rel_l2=rel, rel_l2_max=max(rel), peak_p_rel_err=abs(pka - pkb) / max(pkb, 1e-300),
+                second_harmonic_rel_err=abs(ha - hb) / max(hb, 1e-300), peak_p=pkb)
+
+
 def compare_precisions(pk, p, steps, warmup, mixed_contraction=0, other_contractions=True, timing_steps=20,
-                       until_burst_passed=False):
+                       until_burst_passed=False, full_length_others=()):
     """FP64 and mixed runs of the same problem from t = 0: speed, and accuracy of mixed against
     FP64 at t_end (the mixed run's derivative contraction: 0 VALU, 1 MFMA FP32, 2 MFMA
     split-BF16). With until_burst_passed the runs go on until the whole tone burst has passed
     the on-axis probe (at least `steps`), so that the probe trace holds it; the second harmonic
     is measured there and set beside the plane-wave Fubini estimate. The other two contractions
-    are timed over `timing_steps` and compared with an FP64 run of the same length."""
+    run the same whole trajectory when named in full_length_others; otherwise they are timed over
+    `timing_steps` and compared with an FP64 run of the same length."""
     import torch
     pr = probe_node(p)
     depth = probe_depth(p, pr)
@@ -262,17 +278,12 @@ def compare_precisions(pk, p, steps, warmup, mixed_contraction=0, other_contract
     a = s32.reshape(4, -1)
     b = s64.reshape(4, -1)
     del s32
-    rel = [float(np.linalg.norm(a[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-300)) for f in range(4)]
-    ba = (p.b_over_a / (2 * p.rho0))[:, None]
-    c2 = (p.c0 ** 2)[:, None]
-    r64p = b[0].reshape(p.E, -1)
-    r32p = a[0].reshape(p.E, -1)
-    pk64 = float(np.abs(c2 * (r64p + ba * r64p * r64p)).max())
-    pk32 = float(np.abs(c2 * (r32p + ba * r32p * r32p)).max())
-    del a, r32p
     f0 = p.source.frequency
     dt = r64["dt"]
-    h64, h32 = dft_bin(tr64, dt, 2 * f0), dft_bin(tr32, dt, 2 * f0)
+    acc = _mixed_vs(p, a, b, tr32, tr64, dt, f0)
+    rel = acc["rel_l2"]
+    del a
+    h64 = dft_bin(tr64, dt, 2 * f0)
     f64 = dft_bin(tr64, dt, f0)
     rate64 = p.dof * 5 * steps / r64["elapsed"]
     rate32 = p.dof * 5 * steps / r32["elapsed"]
@@ -285,10 +296,10 @@ def compare_precisions(pk, p, steps, warmup, mixed_contraction=0, other_contract
         fp64_hbm_frac=p.dof * BYTES_FP64 / (r64["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         mixed_hbm_frac=p.dof * BYTES_MIXED / (r32["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         mixed_rel_l2_max=max(rel), mixed_rel_l2=rel,
-        peak_p_fp64=pk64, peak_p_rel_err=abs(pk32 - pk64) / max(pk64, 1e-300),
+        peak_p_fp64=acc["peak_p"], peak_p_rel_err=acc["peak_p_rel_err"],
         probe_depth_m=depth, probe_trace_max_p_fp64=float(np.abs(tr64).max()),
         fundamental_fp64=f64, second_harmonic_fp64=h64, second_harmonic_ratio_fp64=h64 / max(f64, 1e-300),
-        second_harmonic_rel_err=abs(h32 - h64) / max(h64, 1e-300),
+        second_harmonic_rel_err=acc["second_harmonic_rel_err"],
         harmonic_window="Hann over the whole probe trace, t = dt .. t_end",
     )
     if until_burst_passed and p.source.lead is None:
@@ -296,12 +307,25 @@ def compare_precisions(pk, p, steps, warmup, mixed_contraction=0, other_contract
         out.update(fubini_second_harmonic=e2, fubini_ratio=e2 / max(e1, 1e-300), fubini_sigma=sigma,
                    second_harmonic_vs_fubini=h64 / max(e2, 1e-300))
     others = {}
+    for con in full_length_others:
+        ctxm, rm = timed_run(pk, p, pk.dg3d.MIXED, steps, warmup, probe=pr, contraction=con)
+        sm = ctxm.get_state(0).reshape(4, -1)
+        trm = ctxm.get_probes()[:, 0]
+        ctxm.close()
+        accm = _mixed_vs(p, sm, b, trm, tr64, dt, f0)
+        del sm
+        others[CONTRACTION_NAMES[con]] = dict(stage_ms=rm["stage_ms"], variant=[rm["variant"], rm["xcd"]],
+                                              speedup_vs_fp64=r64["stage_ms"] / rm["stage_ms"], steps=total,
+                                              rel_l2_max=accm["rel_l2_max"], rel_l2=accm["rel_l2"],
+                                              peak_p_rel_err=accm["peak_p_rel_err"],
+                                              second_harmonic_rel_err=accm["second_harmonic_rel_err"])
+    del b, s64
     if other_contractions:
         short64, rs = timed_run(pk, p, pk.dg3d.FP64, timing_steps, warmup, variant=r64["variant"], xcd=r64["xcd"])
         ref = short64.get_state(0).reshape(4, -1)
         short64.close()
         for con in range(len(CONTRACTION_NAMES)):
-            if con == mixed_contraction:
+            if con == mixed_contraction or con in full_length_others:
                 continue
             ctxm, rm = timed_run(pk, p, pk.dg3d.MIXED, timing_steps, warmup, contraction=con)
             sm = ctxm.get_state(0).reshape(4, -1)
@@ -570,7 +594,8 @@ def main():
         out["mixed_c3"] = compare_precisions(pk, pk.problem.config_c3(num_elements=args.elements),
                                              args.precision_steps, args.warmup,
                                              mixed_contraction=pk.dg3d.CONTRACT_MFMA_BF16, timing_steps=args.steps,
-                                             until_burst_passed=True)
+                                             until_burst_passed=True,
+                                             full_length_others=(pk.dg3d.CONTRACT_MFMA_F32,))
     if not args.no_extra:
         out["small_c1"] = small_block_run(pk)
         out["other_paths_c2"] = other_paths_run(pk, args.elements, min(args.steps, 200), args.warmup)
