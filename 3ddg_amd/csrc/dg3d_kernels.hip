@@ -267,7 +267,7 @@ __device__ __forceinline__ void column_line(int col, int& fo, int& oo, int& ls) 
 
 // One contraction column's line, mapped once per tile: loads and stores of its node m, which
 // read as 0 and are dropped past the line's end (m >= NN) and past the last column.
-// A P/rho0 column (field 0) reads its line shifted by the line's first value (see
+// A P/rho0 column (field 0) reads its line shifted by the value at the line's middle node (see
 // kShiftPressureLines in stage_body); mass columns read the line as stored.
 template <int NN>
 struct TileLine {
@@ -279,7 +279,7 @@ struct TileLine {
     column_line<NN>(ok ? col : 0, fo, oo, ls);  // branch-free: column 0 stands in past the end
     const int rr = (ok ? col : 0) % (6 * NN * NN);
     const bool pressure = ((rr / (NN * NN)) & 1) == 0;
-    const float v0 = sF[fo];
+    const float v0 = sF[fo + (NN / 2) * ls];
     sh = pressure ? v0 : 0.f;
   }
   __device__ __forceinline__ float load(const float* sF, int m) const {
@@ -294,15 +294,15 @@ struct TileLine {
 
 // The workgroup's derivative contractions as one GEMM on the matrix cores:
 // out[i][col] = sum_m A[i][m] line_col[m], one column per (element, axis, field, face node);
-// P/rho0 lines enter shifted by their first value, as in the VALU contraction of the FP32 stage.
+// P/rho0 lines enter shifted by their middle value, as in the VALU contraction of the FP32 stage.
 // P copies of A sit block-diagonally in the 16x16 operand, block b on rows and K indices
 // [b*BS, b*BS + NN); the block stride BS (4 for NN <= 4, else 8) is a multiple of 4, so the 4
 // rows a lane holds (4g..4g+3, in both the K operand and the result) lie in one block. A tile
 // covers 16 columns per block (columns col0 + 16 b), and waves take tiles round-robin.
 //   CONTRACT 1: v_mfma_f32_16x16x4f32, FP32 operands, 4 K steps.
 //   CONTRACT 2: v_mfma_f32_16x16x16bf16_1k with A and the lines split into three bf16 pieces
-//               each; the six products of combined order <= 2 are accumulated in FP32,
-//               smallest first.
+//               each; the five products of combined order 1 and 2 are accumulated in FP32,
+//               smallest first, and the hi x hi products from zero, then added to them.
 // Lane layouts (16x16 tile, lane l, g = l/16, j = l%16):
 //   16x16x4f32:        A[j][k = g], B[k = g][j], D[4g + r][j]   (K step s covers 4s..4s+3)
 //   16x16x16bf16_1k:   A[j][k = 4g + t], B[k = 4g + t][j], D[4g + r][j]
@@ -367,15 +367,19 @@ __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf,
       split3x2(float2v{ln.load(sF, md), ln.load(sF, md + 1)}, h0, m0, l0);
       split3x2(float2v{ln.load(sF, md + 2), ln.load(sF, md + 3)}, h1, m1, l1);
       const bf16x4 bh = join4(h0, h1), bm = join4(m0, m1), bl = join4(l0, l1);
-      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+      floatx4 acc = zero;
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bm, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bm, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bh, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+      // the dominant hi x hi group from a zero accumulator, then one round-to-nearest add: fed
+      // the small terms as its accumulator input, the matrix core biased the sum (by about half
+      // its mean rounding error, always the same way), which grew linearly over a run
+      const floatx4 hh = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, zero, 0, 0, 0);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) ln.store(sOut, md + r, acc[r]);
+      for (int r = 0; r < 4; ++r) ln.store(sOut, md + r, hh[r] + acc[r]);
     }
   }
 }
@@ -434,8 +438,8 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   static_assert(NT >= EPB * S::ES, "workgroup too small for the element group");
   static_assert(EPB <= 32 && NN * NN * NN <= 512, "face item encoding: local element in 5 bits, node in 9");
   // FP32 stage: every line's P/rho0 values enter the contraction and the flux pair shifted by
-  // the line's first value c (node 0 along the axis), and the neighbour's by c rho0_s/rho0_o,
-  // i.e. the same pressure. A constant pressure drops out of the line's equation exactly: the
+  // the value c at the line's middle node (NN/2 along the axis), and the neighbour's by
+  // c rho0_s/rho0_o, i.e. the same pressure. A constant pressure drops out of the line's equation exactly: the
   // upwind flux's pressure weights sum to one (z1 + z2 over z1 + z2), so it moves both face
   // fluxes by the constant, which cancels the constant's volume term, and the jump terms see
   // no change. In FP32 the shift removes the cancellation between a line's volume term and its
@@ -744,9 +748,9 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     }
     CT* Fl = sFl + ((fle * 6 + dir) * NF2 + fn) * 2;
     if constexpr (kShiftPressureLines) {
-      // the shift of the line through this face node along the face's axis: its node 0
+      // the shift of the line through this face node along the face's axis: its middle node
       const int b = fn / NN, cc = fn - (fn / NN) * NN;
-      const CT c0 = (dir & 1) ? F[face_node<NN>(axis, 0, b, cc)] : mom_s;
+      const CT c0 = F[face_node<NN>(axis, NN / 2, b, cc)];
       mom_o -= c0 * K[12];
       const CT mom_sd = mom_s - c0;
       Fl[0] = K[0] * mom_o + K[1] * mom_sd + K[2] * mass_o + K[3] * mass_s;
@@ -796,7 +800,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
       dp = sOut[(le * 6 + 2 * a) * NP + node];
       dm = sOut[(le * 6 + 2 * a + 1) * NP + node];
     } else {
-      const CT c0 = kShiftPressureLines ? F[base] : CT(0);
+      const CT c0 = kShiftPressureLines ? F[base + (NN / 2) * ls] : CT(0);
 #pragma unroll
       for (int mm = 0; mm < NN; ++mm) {
         const CT av = sA[ix * NN + mm];

@@ -7,7 +7,7 @@ v_mfma_f32_16x16x4f32 and v_mfma_f32_16x16x16bf16_1k, and the round-robin of til
 It checks what the kernel relies on without a bounds check of its own: every LDS read and write
 stays inside its array, every output line is written exactly once, and the tiled GEMM equals
 the direct per-node contraction out[i] = sum_m A[i][m] line[m] along each axis, where a P/rho0
-line enters shifted by its first value (the FP32 stage's shifted pressure lines).
+line enters shifted by its middle value (the FP32 stage's shifted pressure lines).
 """
 import numpy as np
 import pytest
@@ -52,7 +52,8 @@ def emulate(n, epb, mode, A, sF):
         fo, oo, ls = column_line(n, c)
         assert 0 <= fo < sF.size
         pressure = ((c % (6 * NF2)) // NF2) % 2 == 0
-        return ok, fo, oo, ls, (sF[fo] if pressure else 0.0)
+        assert 0 <= fo + (n // 2) * ls < sF.size
+        return ok, fo, oo, ls, (sF[fo + (n // 2) * ls] if pressure else 0.0)
 
     def load(ln, m):
         ok, fo, _, ls, sh = ln
@@ -110,7 +111,8 @@ def direct(n, epb, A, sF):
                 ls = NF2 if a == 0 else (n if a == 1 else 1)
                 base = node - idx[a] * ls
                 F = le * 4 * NP
-                ref[(le * 6 + 2 * a) * NP + node] = sum(A[idx[a], m] * (sF[F + base + m * ls] - sF[F + base])
+                mid = sF[F + base + (n // 2) * ls]
+                ref[(le * 6 + 2 * a) * NP + node] = sum(A[idx[a], m] * (sF[F + base + m * ls] - mid)
                                                         for m in range(n))
                 ref[(le * 6 + 2 * a + 1) * NP + node] = sum(
                     A[idx[a], m] * sF[F + (1 + a) * NP + base + m * ls] for m in range(n))
