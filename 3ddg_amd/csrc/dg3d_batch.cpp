@@ -149,9 +149,11 @@ dg3d::BatchLaunch stage_launch(const dg3d_batch* b, const dg3d_batch::Group& g, 
 }
 
 // Stage schedules: every order on a branch of its own (the orders share the device); all orders
-// one after another on the batch stream, largest first (each has the device to itself); or two
-// branches with about the same number of nodes.
-constexpr int kScheduleBranches = 0, kScheduleSerial = 1, kNumSchedules = 3;  // 2: two balanced branches
+// one after another on the batch stream, largest first (each has the device to itself); two
+// branches with about the same number of nodes; or the largest order on one branch and the
+// others one after another on a second, so that they run beside it and fill the device while its
+// last workgroups drain.
+constexpr int kScheduleBranches = 0, kScheduleSerial = 1, kScheduleTwo = 2, kScheduleBeside = 3, kNumSchedules = 4;
 
 void set_schedule(dg3d_batch* b, int schedule) {
   b->schedule = schedule;
@@ -160,6 +162,7 @@ void set_schedule(dg3d_batch* b, int schedule) {
     auto& g = b->groups[i];
     if (schedule == kScheduleBranches) g.branch = (int)i;
     else if (schedule == kScheduleSerial) g.branch = -1;
+    else if (schedule == kScheduleBeside) g.branch = (i == 0) ? 0 : 1;  // groups are in descending order
     else {  // kScheduleTwo
       // groups are in descending order; each goes to the branch with fewer nodes so far
       g.branch = (load[1] < load[0]) ? 1 : 0;

@@ -46,9 +46,11 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-solve workload")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched 256-solve workload")
     ap.add_argument("--case", choices=["c3m_valu", "c3m_mfma_f32", "c3m_mfma_bf16", "c5m_valu", "c5m_mfma_f32",
-                                       "c5m_mfma_bf16", "c5_fp64", "c2_rk33", "c2_ldg"], default=None,
-                    help="time one workload alone (for profiling): 64^3 N=7 or 48^3 N=5, mixed with the VALU or a "
-                         "matrix-core contraction, or 48^3 N=5 FP64; prints one JSON line")
+                                       "c5m_mfma_bf16", "c5_fp64", "c3_fp64", "n6_fp64", "n6m_valu", "c2_rk33",
+                                       "c2_ldg"], default=None,
+                    help="time one workload alone (for profiling): 64^3 N=7, 48^3 N=5 (stretched two-layer) or "
+                         "48^3 N=6 (the burst workload at that order), FP64 or mixed with the VALU or a "
+                         "matrix-core contraction; 64^3 N=4 with RK33 or LDG diffusion; prints one JSON line")
     ap.add_argument("--c5-steps", type=int, default=2000, help="steps of the combined 48^3 N=5 workload (0: skip)")
     ap.add_argument("--precision-steps", type=int, default=500,
                     help="least steps of the 64^3 N=4 and N=7 precision comparisons (they run on until the burst "
@@ -527,7 +529,12 @@ def main():
         return
     if args.case:
         name, rest = args.case.split("_", 1)
-        p = (pk.problem.config_c3 if name == "c3m" else pk.problem.config_c5)(steps=args.steps)
+        if name in ("c3m", "c3"):
+            p = pk.problem.config_c3(steps=args.steps)
+        elif name in ("n6", "n6m"):
+            p = pk.problem.config_c2(num_elements=48, steps=args.steps, order=6)
+        else:
+            p = pk.problem.config_c5(steps=args.steps)
         if rest == "fp64":
             prec, con, bpn = pk.dg3d.FP64, 0, BYTES_FP64
         else:
