@@ -478,6 +478,21 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   const int node = tid - le * ES;
   const int e = e0 + le;
   const bool io = (le < EPB) && (e < E);
+  // When the element groups lie in z columns (EPB divides nE_z, as groups start at multiples of
+  // EPB), element e0 + l has the coordinates of e0 with k + l: they are found once, on the
+  // scalar unit, instead of by two divisions per face item and helper lane
+  const bool col_groups = (m.nE[2] % EPB) == 0;
+  int gc[3];
+  elem_coords(m, (uint32_t)e0, gc);
+  auto coords_of = [&](int l, int* c) {
+    if (col_groups) {
+      c[0] = gc[0];
+      c[1] = gc[1];
+      c[2] = gc[2] + l;
+    } else {
+      elem_coords(m, (uint32_t)(e0 + l), c);
+    }
+  };
   const bool valid = io && node < NP;
   // lanes past the workgroup's elements load a slot of its last element, a line that its own
   // lanes read anyway, rather than one fixed element that every workgroup would read
@@ -589,7 +604,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   double ms_c = 1.0, ms_r = 1.0, ms_ir = 1.0, mo_c = 1.0, mo_r = 1.0, mo_ba = 0.0, mo_ir = 1.0, hiw = 0.0;
   if (helper && he < E) {
     int ec[3];
-    elem_coords(m, (uint32_t)he, ec);
+    coords_of(hle, ec);
     const int nb = face_neighbour(m, he, ec, hdir);
     const uint32_t ms = (uint32_t)he * kMatStride;
     // the halo element has the adjacent element's material (Block.cpp:167-248)
@@ -651,7 +666,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
         const int on = face_node<NN>(axis, (dir & 1) ? 0 : NN - 1, b, cc);
         item[r] = (unsigned)fle | ((unsigned)dir << 6) | ((unsigned)fn << 9) | ((unsigned)sn << 16);
         int ec[3];
-        elem_coords(m, (uint32_t)he, ec);
+        coords_of(fle, ec);
         const int nb = face_neighbour(m, he, ec, dir);
         const double* tr = trace_ptr[0];
 #pragma unroll
@@ -703,7 +718,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     for (int k = 0; k < kNK; ++k) K[k] = CT(0);
     if (he < E) {
       face_coeffs<CT>(hdir & 1, (CT)ms_c, (CT)ms_r, (CT)ms_ir, (CT)mo_c, (CT)mo_r, (CT)mo_ba, (CT)mo_ir, K);
-      K[12] = (CT)(ms_r / mo_r);  // exactly 1 between elements of equal rho0
+      if constexpr (!std::is_same<CT, double>::value) K[12] = (CT)(ms_r / mo_r);  // 1 for equal rho0
       if ((hdir & 1) == 0) sIw[hle * 3 + 2 - (hdir >> 1)] = (CT)hiw;
     }
 #pragma unroll
