@@ -163,7 +163,7 @@ void set_schedule(dg3d_batch* b, int schedule) {
     if (schedule == kScheduleBranches) g.branch = (int)i;
     else if (schedule == kScheduleSerial) g.branch = -1;
     else if (schedule == kScheduleBeside) g.branch = (i == 0) ? 0 : 1;  // groups are in descending order
-    else {  // kScheduleTwo
+    else if (schedule == kScheduleTwo) {
       // groups are in descending order; each goes to the branch with fewer nodes so far
       g.branch = (load[1] < load[0]) ? 1 : 0;
       load[g.branch] += g.nodes;

@@ -399,7 +399,8 @@ int reduce_partials(dg3d_ctx* c, int per_block, int blocks) {
 // Times every (kernel variant, workgroup order) pair on launches of a stage that reads and writes
 // the full register set (LSRK54 stage 2; RK33 stage 2 with register 0 as both inputs), after a
 // warm-up, in two sweeps of which each pair keeps the faster; the four fastest are timed again in
-// longer alternating sweeps and the fastest on average is kept. The launches write only the work
+// longer alternating sweeps, and the two best of those in alternating windows of about 0.3 s, of
+// which the faster is kept. The launches write only the work
 // registers, which the next step's first stage overwrites without reading, so tuning leaves the
 // solution unchanged.
 int autotune(dg3d_ctx* c, int scheme) {
@@ -410,6 +411,35 @@ int autotune(dg3d_ctx* c, int scheme) {
     hipError_t e = hipMemsetAsync(c->work_du, 0, (c->precision == DG3D_MIXED ? 4 : 8) * 4 * (size_t)c->fs, c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "autotune");
   }
+  // The tuning launches pass the state back and forth between two work registers, as the stages
+  // of a step do (each stage reads what the previous one wrote), starting from a pseudo-random
+  // state with every value a normal number. Launches that re-read one register, or that started
+  // from an initial condition whose pulse tails are zero or subnormal over most of the mesh, ranked
+  // the workgroup orders differently than long runs did (64^3 N=7 split-BF16: order 0 ahead by
+  // 1.5 % on the untouched pulse, 2 % behind once it has spread). For LSRK54 the second register
+  // is a temporary allocation.
+  const size_t reg_bytes = sizeof(double) * (size_t)c->nfields * (size_t)c->fs;
+  double* ping = scheme == DG3D_LSRK54 ? c->work_q : c->work_k1;
+  double* pong = c->work_k2;
+  double* tmp = nullptr;
+  hipError_t e = hipSuccess;
+  if (scheme == DG3D_LSRK54) {
+    e = hipMalloc(reinterpret_cast<void**>(&tmp), reg_bytes);
+    if (e != hipSuccess) return c->hip_fail(e, "autotune");
+    pong = tmp;
+  }
+  struct TmpFree {  // on every return: wait for the launches that use it, then free it
+    double* p;
+    hipStream_t s;
+    ~TmpFree() {
+      if (!p) return;
+      (void)hipStreamSynchronize(s);  // teardown: status unchecked
+      (void)hipFree(p);
+    }
+  } tmp_free{tmp, c->stream};
+  e = dg3d::launch_fill_pattern(ping, 1.0e-3, (int64_t)c->nfields * c->fs, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "autotune");
+  int parity = 0, stage_count = 0;
   auto tuning_launch = [&](int v, int x) {
     dg3d::RhsLaunch L = base_launch(c);
     L.variant = v;
@@ -417,17 +447,21 @@ int autotune(dg3d_ctx* c, int scheme) {
     L.mixed = (c->precision == DG3D_MIXED) ? 1 : 0;
     L.mask = dg3d::kTermAll;
     L.dt = 1e-30;
-    L.q_in = c->regs[0].d;
+    L.q_in = parity ? pong : ping;
+    double* out = parity ? ping : pong;
+    parity ^= 1;
     if (scheme == DG3D_LSRK54) {
+      // the five stages' coefficients in turn, as a step has them (stage 1 skips the du reads)
+      const int st = stage_count++ % 5;
       L.epilogue = dg3d::EPI_LSRK;
-      L.q_out = c->work_q;
+      L.q_out = out;
       L.du = c->work_du;
-      L.a = kLsA[1];
-      L.b = kLsB[1];
+      L.a = kLsA[st];
+      L.b = kLsB[st];
     } else {
       L.epilogue = dg3d::EPI_RK33;
       L.old = c->regs[0].d;
-      L.out = c->work_k2;
+      L.out = out;
       L.a = 0.75;
       L.b = 0.25;
     }
@@ -435,7 +469,6 @@ int autotune(dg3d_ctx* c, int scheme) {
     return L;
   };
   // bring the clocks up before anything is timed
-  hipError_t e = hipSuccess;
   for (int r = 0; r < 8 && e == hipSuccess; ++r) e = dg3d::launch_rhs(tuning_launch(0, c->xcd), c->stream);
   if (e != hipSuccess) return c->hip_fail(e, "autotune");
   // two sweeps over every (variant, order) pair; each pair keeps its faster sweep
@@ -445,10 +478,9 @@ int autotune(dg3d_ctx* c, int scheme) {
   for (int sweep = 0; sweep < 2; ++sweep) {
     for (int v = 0; v < dg3d::kNumVariants; ++v) {
       for (int x = 0; x < dg3d::kNumOrders; ++x) {
-        const dg3d::RhsLaunch L = tuning_launch(v, x);
-        e = dg3d::launch_rhs(L, c->stream);  // first use loads the code object
+        e = dg3d::launch_rhs(tuning_launch(v, x), c->stream);  // first use loads the code object
         if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
-        for (int r = 0; r < reps && e == hipSuccess; ++r) e = dg3d::launch_rhs(L, c->stream);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = dg3d::launch_rhs(tuning_launch(v, x), c->stream);
         if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
         if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
         if (e != hipSuccess) return c->hip_fail(e, "autotune");
@@ -459,20 +491,20 @@ int autotune(dg3d_ctx* c, int scheme) {
       }
     }
   }
-  // the four fastest pairs again, in three alternating sweeps of eight launches, ranked by their
+  // the four fastest pairs again, in three alternating sweeps of sixteen launches, ranked by their
   // mean: at 64^3 N=7 the workgroup orders differ by up to 9 % in long runs, and one noisy first
   // round could pick an order several percent slower than the best
-  constexpr int kFinal = 4, kFinalSweeps = 3, kFinalReps = 8;
+  constexpr int kFinal = 4, kFinalSweeps = 3, kFinalReps = 16;
   std::vector<int> idx(kPairs);
   for (int i = 0; i < kPairs; ++i) idx[i] = i;
   std::partial_sort(idx.begin(), idx.begin() + kFinal, idx.end(), [&](int a, int b) { return best[a] < best[b]; });
   float total[kFinal] = {0.f, 0.f, 0.f, 0.f};
   for (int sweep = 0; sweep < kFinalSweeps; ++sweep) {
     for (int f = 0; f < kFinal; ++f) {
-      const dg3d::RhsLaunch L = tuning_launch(idx[f] / dg3d::kNumOrders, idx[f] % dg3d::kNumOrders);
-      e = dg3d::launch_rhs(L, c->stream);
+      const int fv = idx[f] / dg3d::kNumOrders, fx = idx[f] % dg3d::kNumOrders;
+      e = dg3d::launch_rhs(tuning_launch(fv, fx), c->stream);
       if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
-      for (int r = 0; r < kFinalReps && e == hipSuccess; ++r) e = dg3d::launch_rhs(L, c->stream);
+      for (int r = 0; r < kFinalReps && e == hipSuccess; ++r) e = dg3d::launch_rhs(tuning_launch(fv, fx), c->stream);
       if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
       if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
       if (e != hipSuccess) return c->hip_fail(e, "autotune");
@@ -481,7 +513,30 @@ int autotune(dg3d_ctx* c, int scheme) {
       total[f] += ms;
     }
   }
-  const int k = idx[std::min_element(total, total + kFinal) - total];
+  // the two best of those in long alternating windows of about 0.3 s each: over short windows the
+  // 64^3 N=7 split-BF16 stage ranked workgroup order 0 first on some boxes, while every long run
+  // had order 2 ahead by 6 % (the chip's clocks settle over longer runs than a few launches)
+  int order2[kFinal];
+  for (int f = 0; f < kFinal; ++f) order2[f] = f;
+  std::sort(order2, order2 + kFinal, [&](int a, int b) { return total[a] < total[b]; });
+  const float launch_ms = total[order2[0]] / (kFinalSweeps * kFinalReps);
+  const int long_reps = std::max(16, std::min(512, (int)(300.f / std::max(launch_ms, 1e-3f))));
+  float window[2] = {0.f, 0.f};
+  for (int sweep = 0; sweep < 2; ++sweep) {
+    for (int f = 0; f < 2; ++f) {
+      const int fv = idx[order2[f]] / dg3d::kNumOrders, fx = idx[order2[f]] % dg3d::kNumOrders;
+      e = dg3d::launch_rhs(tuning_launch(fv, fx), c->stream);
+      if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+      for (int r = 0; r < long_reps && e == hipSuccess; ++r) e = dg3d::launch_rhs(tuning_launch(fv, fx), c->stream);
+      if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+      if (e != hipSuccess) return c->hip_fail(e, "autotune");
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+      window[f] += ms;
+    }
+  }
+  const int k = idx[order2[window[1] < window[0] ? 1 : 0]];
   c->tune_variant[scheme] = k / dg3d::kNumOrders;
   c->tune_xcd[scheme] = k % dg3d::kNumOrders;
   return DG3D_OK;

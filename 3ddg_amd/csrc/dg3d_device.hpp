@@ -268,6 +268,9 @@ hipError_t launch_ldg_q(const LdgLaunch& L, hipStream_t s);
 // out = out*a + in*b over nfields fields of length fs.
 hipError_t launch_axpby(double* out, double a, const double* in, double b, int64_t count, hipStream_t s);
 
+// out[i] = pseudo-random normal numbers of magnitude amp/16..amp and either sign, i < count.
+hipError_t launch_fill_pattern(double* out, double amp, int64_t count, hipStream_t s);
+
 // Reductions: partial results per block into `partials` (size returned by reduce_blocks).
 int reduce_blocks(int64_t count);
 hipError_t launch_maxabs(const double* reg, int64_t fs, int nfields, double* partials, hipStream_t s);

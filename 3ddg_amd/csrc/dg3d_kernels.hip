@@ -1209,6 +1209,19 @@ __global__ void k_axpby(double* out, double a, const double* in, double b, int64
     out[i] = out[i] * a + in[i] * b;
 }
 
+// Pseudo-random values in [-amp, amp), all normal numbers (a 64-bit mix of the index).
+__global__ void k_fill_pattern(double* out, double amp, int64_t count) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ull;
+    h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+    h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+    h ^= h >> 31;
+    const double u = (double)(h >> 11) * 0x1.0p-53;  // [0, 1)
+    out[i] = amp * (0.0625 + 0.9375 * u) * ((h & 1) ? 1.0 : -1.0);
+  }
+}
+
 constexpr int kRedThreads = 256;
 constexpr int kRedMaxBlocks = 1024;
 
@@ -1615,6 +1628,14 @@ hipError_t launch_axpby(double* out, double a, const double* in, double b, int64
   int64_t blocks = (count + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_axpby, dim3((unsigned)blocks), dim3(256), 0, s, out, a, in, b, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_pattern(double* out, double amp, int64_t count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill_pattern, dim3((unsigned)blocks), dim3(256), 0, s, out, amp, count);
   return hipGetLastError();
 }
 
