@@ -460,3 +460,48 @@ def test_fused_orders_1_and_7_vs_oracle(order, n_e, scheme, precision):
     print(f"  rel L2 vs oracle {err}", flush=True)
     assert np.linalg.norm(r - st0) > 1e-3 * np.linalg.norm(st0)
     assert err.max() < (TOL_FP64 if precision == dg3d.FP64 else TOL_MIXED), err
+
+
+# elements per workgroup of the stage kernel's two shapes, per order
+_EPB = {1: (32, 16), 2: (8, 16), 3: (4, 8), 4: (2, 4), 5: (2, 1), 6: (2, 1), 7: (1, 2)}
+
+
+@pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6, 7])
+def test_interior_workgroups_vs_oracle(order, scheme):
+    """Element columns away from every block face, whose face neighbours are all gathered from the
+    state, beside columns on the periodic x faces, the exterior faces and the source face: a
+    4 x 4 x 3*EPB block (EPB the larger of the two elements-per-workgroup shapes, so that both
+    shapes have whole columns inside), heterogeneous material, random state, the burst near its
+    peak, 4 steps. Every variant gives the same bits, and they equal the oracle to 1e-12."""
+    print(f"interior_workgroups order={order} scheme={scheme}", flush=True)
+    nE = (4, 4, 3 * max(_EPB[order]))
+    p = heterogeneous(problem._burst_problem("interior", order, nE, 0.75e-3, 4), seed=31)
+    p.periodic = (True, False, False)
+    st0 = _smooth_random_state(p, seed=32)
+    dt = problem.cfl_dt_host(p)
+    t0 = p.source.t0 - 2 * dt
+    ref = None
+    for v in range(8):
+        ctx = problem.make_context(p)
+        ctx.set_state(0, st0)
+        ctx.set_variant(scheme, v, 1)
+        ctx.step(scheme, dt, p.steps, t0)
+        st = ctx.get_state(0)
+        ctx.close()
+        if ref is None:
+            ref = st
+        assert np.array_equal(st, ref), v
+    o = Oracle(p, literal=False, num_registers=4)
+    load_problem_into(o, p, 0)
+    o.set_state(0, st0)
+    o.set_time(t0)
+    for _ in range(p.steps):
+        if scheme == dg3d.RK33:
+            o.rk33(dt, 3, 0, 1, 2)
+            o.copy(0, 3)
+        else:
+            o.lsrk54(dt, 0, 1, 2)
+    err = rel_l2(ref, o.get_state(0))
+    print(f"  rel L2 vs oracle {err}", flush=True)
+    assert err.max() < TOL_FP64, err
