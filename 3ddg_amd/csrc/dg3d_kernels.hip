@@ -1198,7 +1198,8 @@ __global__ void __launch_bounds__((Geo<NN, Shape<NN>::EPB>::NT)) k_ldg_q(const L
 #pragma unroll
     for (int mm = 0; mm < NN; ++mm) sv += sA[ix * NN + mm] * sV[le * NP + base + mm * ls];
     sv += sFl[(le * 6 + dlo) * NF2 + fn] * sL0[ix] + sFl[(le * 6 + dlo + 1) * NF2 + fn] * sLN[ix];
-    L.out[(4 + a) * m.fs + gi] = -(sv * sIw[le * 3 + a]);
+    // nontemporal: the stage that reads Q next ran 3-4 % faster than after cached stores
+    __builtin_nontemporal_store(-(sv * sIw[le * 3 + a]), L.out + (4 + a) * m.fs + gi);
   }
 }
 

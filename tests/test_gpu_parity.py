@@ -505,3 +505,44 @@ def test_interior_workgroups_vs_oracle(order, scheme):
     err = rel_l2(ref, o.get_state(0))
     print(f"  rel L2 vs oracle {err}", flush=True)
     assert err.max() < TOL_FP64, err
+
+
+# elements per workgroup of the LDG Q kernel, per order
+_EPB_LDG = {1: 32, 2: 8, 3: 4, 4: 2, 5: 2, 6: 2, 7: 1}
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6, 7])
+def test_ldg_q_kernel_orders_vs_oracle(order):
+    """LSRK54 with LDG diffusion on a 3 x 3 x 2*EPB block, EPB the Q kernel's elements per
+    workgroup, so that a workgroup's column holds neighbours in z (read from LDS) beside gathered
+    ones; periodic y, heterogeneous D and material, the burst near its peak, 5 steps, under each
+    of the four workgroup orders: the same bits under every order, and equal to the oracle."""
+    print(f"ldg_q_orders order={order}", flush=True)
+    nE = (3, 3, 2 * _EPB_LDG[order])
+    p = heterogeneous(problem._burst_problem("ldg", order, nE, 0.75e-3, 5), seed=41)
+    p.periodic = (False, True, False)
+    rng = np.random.default_rng(42)
+    p.diffusion = 1.0e-3 * (1.0 + rng.random(p.E))
+    st0 = _smooth_random_state(p, seed=43)
+    dt = problem.cfl_dt_host(p)
+    t0 = p.source.t0 - 2 * dt
+    ref = None
+    for x in range(4):
+        ctx = problem.make_context(p, with_q=True, num_registers=1)
+        ctx.set_state(0, st0)
+        ctx.set_variant(dg3d.LSRK54, 0, x)
+        ctx.step(dg3d.LSRK54, dt, p.steps, t0)
+        st = ctx.get_state(0)
+        ctx.close()
+        if ref is None:
+            ref = st
+        assert np.array_equal(st, ref), x
+    o = Oracle(p, literal=False, num_registers=4)
+    load_problem_into(o, p, 0)
+    o.set_state(0, st0)
+    o.set_time(t0)
+    for _ in range(p.steps):
+        o.lsrk54(dt, 0, 1, 2, adv=1, diff=1, sponge=1)
+    err = rel_l2(ref, o.get_state(0))
+    print(f"  rel L2 vs oracle {err}", flush=True)
+    assert err.max() < TOL_FP64, err
