@@ -514,9 +514,10 @@ _EPB_LDG = {1: 32, 2: 8, 3: 4, 4: 2, 5: 2, 6: 2, 7: 1}
 @pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6, 7])
 def test_ldg_q_kernel_orders_vs_oracle(order):
     """LSRK54 with LDG diffusion on a 3 x 3 x 2*EPB block, EPB the Q kernel's elements per
-    workgroup, so that a workgroup's column holds neighbours in z (read from LDS) beside gathered
-    ones; periodic y, heterogeneous D and material, the burst near its peak, 5 steps, under each
-    of the four workgroup orders: the same bits under every order, and equal to the oracle."""
+    workgroup, so that a workgroup's column holds neighbours in z (the LDG stage reads their Q
+    from LDS) beside gathered ones; periodic y, heterogeneous D and material, the burst near its
+    peak, 5 steps, under each of the four workgroup orders: the same bits under every order, and
+    equal to the oracle."""
     print(f"ldg_q_orders order={order}", flush=True)
     nE = (3, 3, 2 * _EPB_LDG[order])
     p = heterogeneous(problem._burst_problem("ldg", order, nE, 0.75e-3, 5), seed=41)
