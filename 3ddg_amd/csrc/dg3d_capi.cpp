@@ -240,6 +240,8 @@ int upload_material(dg3d_ctx* c) {
 
 int ensure_work(dg3d_ctx* c, int scheme) {
   hipError_t e = hipSuccess;
+  const bool fresh_q = scheme == DG3D_LSRK54 && !c->work_q;
+  const bool fresh_k = scheme != DG3D_LSRK54 && !c->work_k1;
   if (scheme == DG3D_LSRK54) {
     if (!c->work_q) e = dmalloc(&c->work_q, (size_t)c->nfields * c->fs);
     if (e == hipSuccess && !c->work_du) {
@@ -251,6 +253,16 @@ int ensure_work(dg3d_ctx* c, int scheme) {
     if (e == hipSuccess && !c->work_k2) e = dmalloc(&c->work_k2, (size_t)c->nfields * c->fs);
   }
   if (e != hipSuccess) return c->fail(DG3D_E_NOMEM, std::string("work register allocation: ") + hipGetErrorString(e));
+  // the kernels write only element slots; the slots past the last element must read as zeros
+  // once a work register becomes register 0 (LSRK54 hands the state over by pointer), since the
+  // reductions cover every slot
+  if (fresh_q || fresh_k) {
+    const size_t reg_bytes = sizeof(double) * (size_t)c->nfields * (size_t)c->fs;
+    if (fresh_q) e = hipMemsetAsync(c->work_q, 0, reg_bytes, c->stream);
+    if (e == hipSuccess && fresh_k) e = hipMemsetAsync(c->work_k1, 0, reg_bytes, c->stream);
+    if (e == hipSuccess && fresh_k) e = hipMemsetAsync(c->work_k2, 0, reg_bytes, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "work register clear");
+  }
   return DG3D_OK;
 }
 
@@ -539,6 +551,9 @@ int autotune(dg3d_ctx* c, int scheme) {
   const int k = idx[order2[window[1] < window[0] ? 1 : 0]];
   c->tune_variant[scheme] = k / dg3d::kNumOrders;
   c->tune_xcd[scheme] = k % dg3d::kNumOrders;
+  // the pattern also filled the slots past the last element, which no stage writes
+  e = hipMemsetAsync(ping, 0, reg_bytes, c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "autotune");
   return DG3D_OK;
 }
 
@@ -754,8 +769,12 @@ int dg3d_set_state(dg3d_ctx* c, int reg, const double* host, int nfields) {
   if (!c || !c->reg_ok(reg) || !host) return c ? c->fail(DG3D_E_INVALID, "set_state: bad register or pointer") : DG3D_E_INVALID;
   if (nfields < 1 || nfields > c->nfields) return c->fail(DG3D_E_INVALID, "set_state: nfields exceeds the register");
   hipError_t e = hipSuccess;
-  if (c->es == c->np && c->fs == c->E * c->np) {
-    e = hipMemcpy(c->regs[reg].d, host, sizeof(double) * nfields * c->fs, hipMemcpyHostToDevice);
+  if (c->es == c->np) {
+    // no padding inside elements: each field is one contiguous run (the slots past the last
+    // element keep their zeros)
+    for (int f = 0; f < nfields && e == hipSuccess; ++f)
+      e = hipMemcpy(c->regs[reg].d + (size_t)f * c->fs, host + (size_t)f * c->E * c->np,
+                    sizeof(double) * c->E * c->np, hipMemcpyHostToDevice);
   } else {
     // host fields are [E][n^3]; the device reserves es slots per element (zeros in the padding)
     std::vector<double> stage((size_t)nfields * c->fs, 0.0);
@@ -774,8 +793,10 @@ int dg3d_get_state(dg3d_ctx* c, int reg, double* host, int nfields) {
   if (nfields < 1 || nfields > c->nfields) return c->fail(DG3D_E_INVALID, "get_state: nfields exceeds the register");
   hipError_t e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) {
-    if (c->es == c->np && c->fs == c->E * c->np) {
-      e = hipMemcpy(host, c->regs[reg].d, sizeof(double) * nfields * c->fs, hipMemcpyDeviceToHost);
+    if (c->es == c->np) {
+      for (int f = 0; f < nfields && e == hipSuccess; ++f)
+        e = hipMemcpy(host + (size_t)f * c->E * c->np, c->regs[reg].d + (size_t)f * c->fs,
+                      sizeof(double) * c->E * c->np, hipMemcpyDeviceToHost);
     } else {
       std::vector<double> stage((size_t)nfields * c->fs);
       e = hipMemcpy(stage.data(), c->regs[reg].d, sizeof(double) * stage.size(), hipMemcpyDeviceToHost);

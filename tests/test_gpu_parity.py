@@ -547,3 +547,23 @@ def test_ldg_q_kernel_orders_vs_oracle(order):
     err = rel_l2(ref, o.get_state(0))
     print(f"  rel L2 vs oracle {err}", flush=True)
     assert err.max() < TOL_FP64, err
+
+
+@pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
+@pytest.mark.parametrize("order", [1, 4])
+def test_register_slots_past_the_elements_stay_zero(scheme, order):
+    """The slots between the last element and the next field (the field pitch is skewed away from
+    powers of two) are never written by a stage, yet the reductions cover them. A zero state with
+    no source, autotuned (the tuner streams a pseudo-random state through the work registers)
+    and stepped (LSRK54 hands the state over by pointer, so register 0 becomes a work register):
+    the max |.| over every slot stays exactly zero."""
+    print(f"slots_past_elements scheme={scheme} order={order}", flush=True)
+    p = small_c2(n_e=3, order=order, steps=3)
+    p.source = problem.Source()
+    ctx = problem.make_context(p)
+    ctx.set_state(0, np.zeros(4 * p.dof))
+    ctx.autotune(scheme)
+    ctx.step(scheme, problem.cfl_dt_host(p), p.steps, 0.0)
+    m = ctx.reduce(dg3d.REDUCE_MAXABS, 0)
+    ctx.close()
+    assert np.all(np.asarray(m)[:4] == 0.0), m

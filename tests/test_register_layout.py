@@ -20,7 +20,7 @@ def elem_stride(n):
 
 
 def field_slots(E, n):
-    return (E * elem_stride(n) + 31) // 32 * 32
+    return (E * elem_stride(n) + 2047) // 2048 * 2048 + 256
 
 
 def layout(n, fp64):
