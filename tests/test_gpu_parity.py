@@ -567,3 +567,30 @@ def test_register_slots_past_the_elements_stay_zero(scheme, order):
     m = ctx.reduce(dg3d.REDUCE_MAXABS, 0)
     ctx.close()
     assert np.all(np.asarray(m)[:4] == 0.0), m
+
+
+@pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
+def test_autotune_mid_run_leaves_the_solution(scheme):
+    """dg3d_autotune between steps times its candidates on work registers only (from a
+    pseudo-random state, which it clears afterwards): 6 steps, a tuning, the same pair pinned
+    again and 6 more steps give the bits of 12 steps in one go (heterogeneous material, B/A=5,
+    the burst source on)."""
+    print(f"autotune_mid_run scheme={scheme}", flush=True)
+    p = heterogeneous(small_c2(n_e=4, order=3, steps=12), seed=51)
+    dt = problem.cfl_dt_host(p)
+    t0 = p.source.t0 - 6 * dt
+    ctx = problem.make_context(p)
+    ctx.set_variant(scheme, 2, 1)
+    ctx.step(scheme, dt, 12, t0)
+    ref = ctx.get_state(0)
+    ctx.close()
+    ctx = problem.make_context(p)
+    ctx.set_variant(scheme, 2, 1)
+    t = ctx.step(scheme, dt, 6, t0)
+    ctx.autotune(scheme)
+    ctx.set_variant(scheme, 2, 1)
+    ctx.step(scheme, dt, 6, t)
+    st = ctx.get_state(0)
+    ctx.close()
+    assert np.linalg.norm(ref) > 0
+    assert np.array_equal(st, ref)
