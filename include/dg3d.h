@@ -170,8 +170,9 @@ int dg3d_stage(dg3d_ctx* ctx, int scheme, int stage, double dt, double t);
 /* Autotuner of the fused stage kernel (no counterpart in the reference): times every kernel
  * variant (0..7: elements per workgroup, occupancy target, RK-register loads issued with the
  * node's own loads or after the first barrier) under every workgroup order
- * (0..3, as xcd_remap above) on launches with a later stage's coefficients that write only the
- * work registers, and keeps the fastest pair for `scheme`. dg3d_step runs it on its first call
+ * (0..3, as xcd_remap above) on stage launches that pass a pseudo-random state between two
+ * work registers, through the scheme's stage coefficients in turn, and keeps the fastest pair
+ * for `scheme`; register 0 and the solution are left unchanged. dg3d_step runs it on its first call
  * unless a pair was pinned with dg3d_set_variant. With the VALU contraction the results do not
  * depend on the pair (the per-node arithmetic is the same). */
 int dg3d_autotune(dg3d_ctx* ctx, int scheme, int* variant, int* xcd_remap);
