@@ -29,6 +29,13 @@ HBM_PEAK_GBS = 8000.0
 BYTES_FP64 = (96.0 + 4 * 128.0) / 5.0   # 121.6
 BYTES_MIXED = (80.0 + 4 * 96.0) / 5.0   # 92.8
 
+_T0 = time.time()
+
+
+def progress(msg):
+    """One line on stderr as each part of the run ends (stdout carries only the JSON line)."""
+    print(f"bench [{time.time() - _T0:6.1f} s] {msg}", file=sys.stderr, flush=True)
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -121,6 +128,8 @@ def timed_run(pk, p, precision, steps, warmup, probe=None, variant=-1, xcd=1, co
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     stage_ms, launches = ctx.last_step_timing()
+    progress(f"  {p.name} {'FP64' if precision == pk.dg3d.FP64 else 'mixed'} contraction {contraction}: "
+             f"{steps} timed steps, stage {stage_ms:.4f} ms")
     return ctx, dict(elapsed=el, stage_ms=stage_ms, launches=launches, dt=dt, t_end=t, variant=variant, xcd=xcd)
 
 
@@ -558,6 +567,7 @@ def main():
     ctx, r = timed_run(pk, p, pk.dg3d.FP64, args.steps, args.warmup, variant=args.variant, xcd=args.xcd)
     maxabs = ctx.reduce(pk.dg3d.REDUCE_MAXABS, 0)
     ctx.close()
+    progress(f"headline 64^3 N={p.order} FP64: {args.steps} steps, stage {r['stage_ms']:.4f} ms")
     if not np.all(np.isfinite(maxabs[:4])):
         print("non-finite state after the run", file=sys.stderr)
         sys.exit(3)
@@ -598,11 +608,13 @@ def main():
                 stage_frac_of_ceiling=achieved / sc["stream_GBs"],
                 pattern=f"{sc['arrays_read']} FP64 arrays read and {sc['arrays_written']} written, nontemporal, "
                         f"256-lane workgroups streaming {sc['chunk']}-value chunks")
+        progress("streaming probe done")
         # the precision comparisons run from t = 0 until the whole burst has passed the probe
         # (at least the workloads' 500 steps), whatever --steps is
         out["mixed_c2"] = compare_precisions(pk, pk.problem.config_c2(num_elements=args.elements),
                                              args.precision_steps, args.warmup, timing_steps=args.steps,
                                              until_burst_passed=True)
+        progress("precision comparison 64^3 N=4 done")
         # the N=7 precision workload runs its derivative contraction as split-BF16 MFMA GEMMs;
         # the FP32 MFMA and VALU contractions run the same whole trajectory beside it
         out["mixed_c3"] = compare_precisions(pk, pk.problem.config_c3(num_elements=args.elements),
@@ -610,11 +622,15 @@ def main():
                                              mixed_contraction=pk.dg3d.CONTRACT_MFMA_BF16, timing_steps=args.steps,
                                              until_burst_passed=True,
                                              full_length_others=(pk.dg3d.CONTRACT_MFMA_F32, pk.dg3d.CONTRACT_VALU))
+        progress("precision comparison 64^3 N=7 done")
     if not args.no_extra:
         out["small_c1"] = small_block_run(pk)
+        progress("8^3 N=3 block done")
         out["other_paths_c2"] = other_paths_run(pk, args.elements, min(args.steps, 200), args.warmup)
+        progress("RK33 and LDG paths done")
     if not args.no_batch and not args.no_extra:
         out["batch_c4"] = batch_run(pk, args.batch_steps)
+        progress("256-solve batch done")
     if not args.no_extra and args.c5_steps > 0:
         # combined workload: stretched two-layer 48^3 N=5 with a focused burst; mixed with the
         # split-BF16 matrix-core contraction against FP64
@@ -622,8 +638,10 @@ def main():
                                                 args.c5_steps - args.warmup, args.warmup,
                                                 mixed_contraction=pk.dg3d.CONTRACT_MFMA_BF16,
                                                 other_contractions=False)
+        progress("48^3 N=5 combined workload done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pk.problem.config_c2(num_elements=args.elements), args.cpu_steps)
+        progress("CPU oracle baseline done")
         cores = out["cpu_baseline"]["cores"]
         out["cpu_literal_rk33"] = cpu_literal_baseline(pk, (args.elements,) * 3, 4, 0.75e-3, cores,
                                                        max(1, args.cpu_steps // 2))
