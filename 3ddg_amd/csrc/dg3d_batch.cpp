@@ -492,11 +492,11 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
       if (src_out) L.ext.trace[d.source_face] = src_out;
       recs[(size_t)i * dg3d::kBatchRecords + r] = L;
     }
-    // initial state: host [4][E][np] -> padded slots
+    // initial state: host [4][E][np] -> element-blocked slots (dg3d::reg_slot), padding zero
     stage.assign((size_t)4 * h.fs, 0.0);
     for (int f = 0; f < 4; ++f)
       for (int64_t el = 0; el < h.E; ++el)
-        std::memcpy(&stage[(size_t)f * h.fs + (size_t)el * h.es], d.state + ((size_t)f * h.E + el) * h.np,
+        std::memcpy(&stage[dg3d::reg_slot(h.fs, h.es, f, el, 0)], d.state + ((size_t)f * h.E + el) * h.np,
                     sizeof(double) * h.np);
     e = hipMemcpy(b->arena[0] + h.slot0, stage.data(), sizeof(double) * stage.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(DG3D_E_HIP, std::string("batch: state upload: ") + hipGetErrorString(e));
@@ -648,7 +648,7 @@ int dg3d_batch_get_state(dg3d_batch* b, int solve, double* host) {
   if (e != hipSuccess) return b->hip_fail(e, "batch get_state");
   for (int f = 0; f < 4; ++f)
     for (int64_t el = 0; el < h.E; ++el)
-      std::memcpy(host + ((size_t)f * h.E + el) * h.np, &stage[(size_t)f * h.fs + (size_t)el * h.es],
+      std::memcpy(host + ((size_t)f * h.E + el) * h.np, &stage[dg3d::reg_slot(h.fs, h.es, f, el, 0)],
                   sizeof(double) * h.np);
   return DG3D_OK;
 }

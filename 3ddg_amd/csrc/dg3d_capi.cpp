@@ -266,6 +266,44 @@ int ensure_work(dg3d_ctx* c, int scheme) {
   return DG3D_OK;
 }
 
+// Fields 0 .. nfields-1 between the host layout ([field][E][n^3]) and a register's
+// element-blocked slots (dg3d::reg_slot), in chunks of elements: a chunk's state fields are one
+// contiguous run of the register, and so are its Q fields. Writing only some of a block's fields
+// reads the chunk first; the padding slots of fully written blocks are written as zeros.
+hipError_t transfer_state(const dg3d_ctx* c, double* dev, double* host, int nfields, bool to_device) {
+  const int es = c->es, np = c->np;
+  const int64_t chunk = std::max<int64_t>(1, ((int64_t)1 << 22) / (4 * es));
+  std::vector<double> buf;
+  hipError_t e = hipSuccess;
+  // fields f_lo .. f_lo+f_n-1 held blk to an element block, the blocks starting at slot d_base
+  auto part = [&](double* d_base, int blk, int f_lo, int f_n, int64_t e0, int64_t e1) {
+    const size_t count = (size_t)(e1 - e0) * blk * es;
+    double* d = d_base + (size_t)e0 * blk * es;
+    if (to_device && f_n == blk) {
+      buf.assign(count, 0.0);
+    } else {
+      buf.resize(count);
+      e = hipMemcpy(buf.data(), d, sizeof(double) * count, hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return;
+    }
+    for (int64_t el = e0; el < e1; ++el)
+      for (int k = 0; k < f_n; ++k) {
+        double* slot = &buf[((size_t)(el - e0) * blk + k) * es];
+        double* h = host + ((size_t)(f_lo + k) * c->E + el) * np;
+        if (to_device) std::memcpy(slot, h, sizeof(double) * np);
+        else std::memcpy(h, slot, sizeof(double) * np);
+      }
+    if (to_device) e = hipMemcpy(d, buf.data(), sizeof(double) * count, hipMemcpyHostToDevice);
+  };
+  const int sf = std::min(nfields, 4), qf = nfields - sf;
+  for (int64_t e0 = 0; e0 < c->E && e == hipSuccess; e0 += chunk) {
+    const int64_t e1 = std::min(c->E, e0 + chunk);
+    part(dev, 4, 0, sf, e0, e1);
+    if (e == hipSuccess && qf > 0) part(dev + 4 * c->fs, 3, 4, qf, e0, e1);
+  }
+  return e;
+}
+
 dg3d::RhsLaunch base_launch(const dg3d_ctx* c) {
   dg3d::RhsLaunch L{};
   L.n = c->n;
@@ -768,22 +806,7 @@ int dg3d_set_material(dg3d_ctx* c, const double* c0, const double* rho0, const d
 int dg3d_set_state(dg3d_ctx* c, int reg, const double* host, int nfields) {
   if (!c || !c->reg_ok(reg) || !host) return c ? c->fail(DG3D_E_INVALID, "set_state: bad register or pointer") : DG3D_E_INVALID;
   if (nfields < 1 || nfields > c->nfields) return c->fail(DG3D_E_INVALID, "set_state: nfields exceeds the register");
-  hipError_t e = hipSuccess;
-  if (c->es == c->np) {
-    // no padding inside elements: each field is one contiguous run (the slots past the last
-    // element keep their zeros)
-    for (int f = 0; f < nfields && e == hipSuccess; ++f)
-      e = hipMemcpy(c->regs[reg].d + (size_t)f * c->fs, host + (size_t)f * c->E * c->np,
-                    sizeof(double) * c->E * c->np, hipMemcpyHostToDevice);
-  } else {
-    // host fields are [E][n^3]; the device reserves es slots per element (zeros in the padding)
-    std::vector<double> stage((size_t)nfields * c->fs, 0.0);
-    for (int f = 0; f < nfields; ++f)
-      for (int64_t el = 0; el < c->E; ++el)
-        std::memcpy(&stage[(size_t)f * c->fs + (size_t)el * c->es], host + ((size_t)f * c->E + el) * c->np,
-                    sizeof(double) * c->np);
-    e = hipMemcpy(c->regs[reg].d, stage.data(), sizeof(double) * stage.size(), hipMemcpyHostToDevice);
-  }
+  const hipError_t e = transfer_state(c, c->regs[reg].d, const_cast<double*>(host), nfields, true);
   if (e != hipSuccess) return c->hip_fail(e, "set_state");
   return DG3D_OK;
 }
@@ -792,20 +815,7 @@ int dg3d_get_state(dg3d_ctx* c, int reg, double* host, int nfields) {
   if (!c || !c->reg_ok(reg) || !host) return c ? c->fail(DG3D_E_INVALID, "get_state: bad register or pointer") : DG3D_E_INVALID;
   if (nfields < 1 || nfields > c->nfields) return c->fail(DG3D_E_INVALID, "get_state: nfields exceeds the register");
   hipError_t e = hipStreamSynchronize(c->stream);
-  if (e == hipSuccess) {
-    if (c->es == c->np) {
-      for (int f = 0; f < nfields && e == hipSuccess; ++f)
-        e = hipMemcpy(host + (size_t)f * c->E * c->np, c->regs[reg].d + (size_t)f * c->fs,
-                      sizeof(double) * c->E * c->np, hipMemcpyDeviceToHost);
-    } else {
-      std::vector<double> stage((size_t)nfields * c->fs);
-      e = hipMemcpy(stage.data(), c->regs[reg].d, sizeof(double) * stage.size(), hipMemcpyDeviceToHost);
-      for (int f = 0; f < nfields && e == hipSuccess; ++f)
-        for (int64_t el = 0; el < c->E; ++el)
-          std::memcpy(host + ((size_t)f * c->E + el) * c->np, &stage[(size_t)f * c->fs + (size_t)el * c->es],
-                      sizeof(double) * c->np);
-    }
-  }
+  if (e == hipSuccess) e = transfer_state(c, c->regs[reg].d, host, nfields, false);
   if (e != hipSuccess) return c->hip_fail(e, "get_state");
   return DG3D_OK;
 }
@@ -1061,9 +1071,9 @@ bool reduce_shape(int kind, int* per_block, int* combine) {
 }
 
 hipError_t launch_reduction(dg3d_ctx* c, int kind, int reg, int reg2, double* part) {
-  if (kind == DG3D_REDUCE_MAXABS) return dg3d::launch_maxabs(c->regs[reg].d, c->fs, 4, part, c->stream);
+  if (kind == DG3D_REDUCE_MAXABS) return dg3d::launch_maxabs(c->regs[reg].d, c->fs, c->es, part, c->stream);
   if (kind == DG3D_REDUCE_ENERGY) return dg3d::launch_energy(c->regs[reg].d, c->mesh(), c->n, c->d_wq, part, c->stream);
-  return dg3d::launch_l2diff(c->regs[reg].d, c->regs[reg2].d, c->fs, 4, part, c->stream);
+  return dg3d::launch_l2diff(c->regs[reg].d, c->regs[reg2].d, c->fs, c->es, part, c->stream);
 }
 
 // The outputs of a reduction kind from its combined values (v: 4 maxima, the 2 energy sums, or
@@ -1170,7 +1180,7 @@ int dg3d_set_probes(dg3d_ctx* c, int count, const int64_t* nodes, int capacity_s
   std::vector<int64_t> slots(count);
   for (int i = 0; i < count; ++i) {
     if (nodes[i] < 0 || nodes[i] >= c->E * c->np) return c->fail(DG3D_E_INVALID, "probe node out of range");
-    slots[i] = (nodes[i] / c->np) * c->es + nodes[i] % c->np;  // host numbering -> device slot
+    slots[i] = dg3d::reg_slot(c->fs, c->es, 0, nodes[i] / c->np, (int)(nodes[i] % c->np));  // host -> rho' slot
   }
   if (dmalloc(&c->d_probe_nodes, count) != hipSuccess ||
       dmalloc(&c->d_probe_buf, (size_t)count * std::max(capacity_steps, 1)) != hipSuccess)

@@ -129,22 +129,29 @@ __host__ __device__ inline uint32_t map_group(uint32_t w, const GroupMap& g) {
   return w;
 }
 
-// Device layout of a field: element-major, with ES(n) node slots per element. n^3 is rounded
-// up to a multiple of 32 slots (n = 3, 5, 6, 7), so every element of an FP64 or FP32 field
-// starts on a 128-byte line and a workgroup's elements cover whole lines: no line is written
-// in part by two workgroups. The padding slots hold zeros. n = 2 keeps 8 slots: its
-// workgroups take 16 or 32 elements, which are whole lines too. The fields of a register are
-// fs slots apart, with fs = round_up(E * ES, 2048) + 256: 2 KiB more than a multiple of 16 KiB
-// (FP64). The stage streams every field at the same element offsets at once, and fields a
-// power of two apart (64^3 elements of 128 or 512 slots give exactly 2^25 or 2^27) put those
-// streams on the same DRAM banks: a probe streaming 8 such arrays in and 8 out ran at 62-63 % of
-// the HBM peak with a 2^25-value pitch and at 75-77 % with any pitch of 256 + k * 2048 values
-// (pitches that are multiples of 1024 values: 64-67 %). The slots past the last element hold
-// zeros.
+// Device layout of a register: element-blocked. Each element has ES(n) node slots per field:
+// n^3 rounded up to a multiple of 32 slots (n = 3, 5, 6, 7), so every element's field starts on a
+// 128-byte line and a workgroup's elements cover whole lines: no line is written in part by two
+// workgroups. The padding slots hold zeros. n = 2 keeps 8 slots: its workgroups take 16 or 32
+// elements, which are whole lines too.
+//   * The state fields rho', u, v, w of element e are one block of 4 ES slots: field f at
+//     (4 e + f) ES + node. A workgroup's elements are then one contiguous run of the register,
+//     which it reads and writes as one stream: a probe of the LSRK stage's pattern (state in,
+//     RK register in place, state out; 64^3 N=4 FP64) streamed 76 % of the HBM peak in this
+//     layout, 72 % with every field its own array (and 68-69 % with the four fields of a node
+//     side by side, 32 bytes per lane).
+//   * The LDG fields Q_x, Q_y, Q_z of a 7-field register follow in blocks of 3 ES slots from
+//     slot 4 fs: field 4 + a at 4 fs + (3 e + a) ES + node.
+//   * A register takes nfields * fs slots, fs = round_up(E * ES, 2048) + 256 (the slots past the
+//     last element hold zeros).
 constexpr int elem_stride(int n) {
   return (n == 2 || (n * n * n) % 32 == 0) ? n * n * n : (n * n * n + 31) / 32 * 32;
 }
 inline int64_t field_slots(int64_t E, int n) { return (E * elem_stride(n) + 2047) / 2048 * 2048 + 256; }
+// Slot of field f (0..6) of element e, node slot `node`, in a register of field unit fs.
+__host__ __device__ inline int64_t reg_slot(int64_t fs, int es, int f, int64_t e, int node) {
+  return f < 4 ? (e * 4 + f) * es + node : 4 * fs + (e * 3 + (f - 4)) * es + node;
+}
 
 constexpr int kMatStride = 8;  // per-element material record (doubles)
 enum MatSlot { MAT_C = 0, MAT_RHO0 = 1, MAT_BA = 2, MAT_SIGMA = 3, MAT_IRHO0 = 4, MAT_DIFF = 5 };
@@ -153,7 +160,7 @@ struct MeshArgs {
   int nE[3];
   int periodic[3];
   int64_t E;          // interior elements (< 2^31)
-  int64_t fs;         // field stride in a register: field_slots(E, n)
+  int64_t fs;         // field unit of a register: field_slots(E, n) (layout: reg_slot)
   int es;             // node slots per element: elem_stride(n)
   int64_t stride[3];  // element index stride per axis: e = (i*nE1 + j)*nE2 + k
   Div32 div12;        // divide by nE1*nE2
@@ -277,11 +284,12 @@ hipError_t launch_axpby(double* out, double a, const double* in, double b, int64
 // out[i] = pseudo-random normal numbers of magnitude amp/16..amp and either sign, i < count.
 hipError_t launch_fill_pattern(double* out, double amp, int64_t count, hipStream_t s);
 
-// Reductions: partial results per block into `partials` (size returned by reduce_blocks).
+// Reductions: partial results per block into `partials` (size returned by reduce_blocks). The
+// max |.| and L2 reductions cover every slot of the four state fields' region (4 fs slots, the
+// slots past the last element included) of a register with es slots per element.
 int reduce_blocks(int64_t count);
-hipError_t launch_maxabs(const double* reg, int64_t fs, int nfields, double* partials, hipStream_t s);
-hipError_t launch_l2diff(const double* a, const double* b, int64_t fs, int nfields, double* partials,
-                         hipStream_t s);
+hipError_t launch_maxabs(const double* reg, int64_t fs, int es, double* partials, hipStream_t s);
+hipError_t launch_l2diff(const double* a, const double* b, int64_t fs, int es, double* partials, hipStream_t s);
 hipError_t launch_energy(const double* reg, const MeshArgs& m, int n, const double* wq, double* partials,
                          hipStream_t s);
 hipError_t launch_cfl(const MeshArgs& m, double dxi_half, double* partials, hipStream_t s);

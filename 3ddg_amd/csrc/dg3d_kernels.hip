@@ -497,41 +497,32 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   // lanes past the workgroup's elements load a slot of its last element, a line that its own
   // lanes read anyway, rather than one fixed element that every workgroup would read
   const int ge = io ? e : min(e0 + EPB - 1, E - 1);
-  const uint32_t gi = (uint32_t)ge * ES + node;
-  const uint32_t fb64 = (uint32_t)(fs * 8);
-  Rsrc rq[4];
-#pragma unroll
-  for (int f = 0; f < 4; ++f) rq[f] = make_rsrc(L.q_in + f * fs, fb64);
+  // Every register access of the workgroup goes through a buffer resource over its own elements'
+  // blocks (elements e0 .. e0+ne-1, contiguous in the element-blocked layout), so the offsets
+  // are small whatever the register's size. gi: the lane's field-0 slot in that run; field f is
+  // ES slots further on.
+  const int ne = (E - e0 < EPB) ? E - e0 : EPB;
+  const uint32_t gi = (uint32_t)((ge - e0) * 4 * ES + node);
+  const uint32_t wb64 = (uint32_t)(ne * 4 * ES * 8);
+  const size_t wofs = (size_t)e0 * 4 * ES;  // the run's first slot
+  const Rsrc rq = make_rsrc(L.q_in + wofs, wb64);
   double q[4], pre[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int f = 0; f < 4; ++f) q[f] = bload<double>(rq[f], gi);
-  // The LSRK register is never read across a face, so its layout is free. A lane can move its
-  // node's share in 16-byte accesses, addressed from the workgroup's first element so that the
-  // offsets stay small: the FP64 register pair-major (rho',u then v,w, two accesses into two
-  // arrays), the FP32 register node-major (four fields in one access). Otherwise it is
-  // field-major like the state. Measured per shape: at 7 nodes per line both 16-byte forms ran
-  // 4-6 % slower than field-major; at every other order they are faster or even
+  for (int f = 0; f < 4; ++f) q[f] = bload<double>(rq, gi + f * ES);
+  // The LSRK register is never read across a face, so its layout within an element's block is
+  // free. A lane can move its node's share in 16-byte accesses: the FP64 register pair-major
+  // (an element's (rho', u) pairs, then its (v, w) pairs: two accesses), the FP32 register
+  // node-major (four fields in one access). Otherwise it is laid out like the state. Measured per
+  // shape: at 7 nodes per line both 16-byte forms ran 4-6 % slower; at every other order they
+  // are faster or even
   constexpr bool kDuNodeMajor = NN != 7 && std::is_same<DUT, float>::value;
   constexpr bool kDuPairMajor = NN != 7 && std::is_same<DUT, double>::value;
-  constexpr bool kDu16 = kDuNodeMajor || kDuPairMajor;
-  Rsrc rp[4], rdu, rdu1;
+  Rsrc rdu;
   uint32_t du_off = 0;
   if constexpr (EPI == EPI_LSRK) {
-    if constexpr (kDu16) {
-      const int ne = (E - e0 < EPB) ? E - e0 : EPB;
-      const uint32_t wbytes = (uint32_t)(ne * ES * 16);
-      const DUT* d0 = static_cast<const DUT*>(L.du);
-      if constexpr (kDuNodeMajor) {
-        rdu = make_rsrc(d0 + (size_t)e0 * ES * 4, wbytes);
-      } else {
-        rdu = make_rsrc(d0 + (size_t)e0 * ES * 2, wbytes);
-        rdu1 = make_rsrc(d0 + 2 * (size_t)fs + (size_t)e0 * ES * 2, wbytes);
-      }
-      du_off = (uint32_t)((ge - e0) * ES + node) * 16u;
-    } else {
-#pragma unroll
-      for (int f = 0; f < 4; ++f) rp[f] = make_rsrc(static_cast<const DUT*>(L.du) + f * fs, (uint32_t)(fs * sizeof(DUT)));
-    }
+    rdu = make_rsrc(static_cast<const DUT*>(L.du) + wofs, (uint32_t)(ne * 4 * ES * sizeof(DUT)));
+    if constexpr (kDuNodeMajor) du_off = (uint32_t)((ge - e0) * ES + node) * 16u;
+    if constexpr (kDuPairMajor) du_off = (uint32_t)((ge - e0) * 2 * ES + node) * 16u;  // pair 1: + ES * 16
   }
   auto load_pre = [&]() {
     if constexpr (EPI == EPI_LSRK) {
@@ -540,17 +531,18 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
           bload4f<kPolicyNT>(rdu, du_off, pre);
         } else if constexpr (kDuPairMajor) {
           bload2d<kPolicyNT>(rdu, du_off, pre[0], pre[1]);
-          bload2d<kPolicyNT>(rdu1, du_off, pre[2], pre[3]);
+          bload2d<kPolicyNT>(rdu, du_off + (uint32_t)ES * 16u, pre[2], pre[3]);
         } else {
 #pragma unroll
-          for (int f = 0; f < 4; ++f) pre[f] = (double)bload<DUT, kPolicyNT>(rp[f], gi);
+          for (int f = 0; f < 4; ++f) pre[f] = (double)bload<DUT, kPolicyNT>(rdu, gi + f * ES);
         }
       }
     }
     if constexpr (EPI == EPI_RK33) {
       if (L.a != 0.0) {
+        const Rsrc ro = make_rsrc(L.old + wofs, wb64);
 #pragma unroll
-        for (int f = 0; f < 4; ++f) pre[f] = bload<double, kPolicyNT>(make_rsrc(L.old + f * fs, fb64), gi);
+        for (int f = 0; f < 4; ++f) pre[f] = bload<double, kPolicyNT>(ro, gi + f * ES);
       }
     }
   };
@@ -578,15 +570,14 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   const CT c = (CT)mat[MAT_C], r0 = (CT)mat[MAT_RHO0];
   const CT ba = (CT)mat[MAT_BA], sg = (CT)mat[MAT_SIGMA];
   const CT ir0 = (CT)mat[MAT_IRHO0];
-  Rsrc rqq[3];
   double qq[3] = {0.0, 0.0, 0.0};
   CT dcoef = CT(0);
   if constexpr (LDG) {
+    // the Q fields: blocks of 3 ES slots from slot 4 fs
+    const Rsrc rqq = make_rsrc(L.q_in + 4 * fs + (size_t)e0 * 3 * ES, (uint32_t)(ne * 3 * ES * 8));
+    const uint32_t qi = (uint32_t)((ge - e0) * 3 * ES + node);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      rqq[a] = make_rsrc(L.q_in + (4 + a) * fs, fb64);
-      qq[a] = bload<double>(rqq[a], gi);
-    }
+    for (int a = 0; a < 3; ++a) qq[a] = bload<double>(rqq, qi + a * ES);
     dcoef = (CT)mat[MAT_DIFF];
   }
 
@@ -682,14 +673,16 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
         const bool local = inner && nb >= e0 && nb < e0 + EPB;
         item[r] |= local ? (kItemLocal | ((unsigned)(nb - e0) << 25)) : 0u;
         if (!local) {
-          const int64_t oi = inner ? (int64_t)nb * ES + on : (int64_t)he * ES + sn;
+          const int64_t oe = inner ? nb : he;  // the element whose node is read
+          const int onode = inner ? on : sn;
+          const int64_t oi = oe * 4 * ES + onode;
           const int64_t ti = ((int64_t)(-1 - nb) * NF2 + fn) * 3;
           const double* prho = state ? L.q_in + oi : tr + ti;
-          const double* pu = state ? L.q_in + (1 + axis) * fs + oi : tr + ti + 1;
+          const double* pu = state ? L.q_in + oi + (1 + axis) * ES : tr + ti + 1;
           nrho[r] = *prho;
           nu[r] = *pu;
           if constexpr (LDG) {
-            const double* pq = state ? L.q_in + (4 + axis) * fs + oi : tr + ti + 2;
+            const double* pq = state ? L.q_in + 4 * fs + (oe * 3 + axis) * ES + onode : tr + ti + 2;
             nq[r] = *pq;
           }
         }
@@ -861,25 +854,24 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     touch[0] = (mask & (kTermSponge | 0x2Au)) != 0u;
 #pragma unroll
     for (int a = 0; a < 3; ++a) touch[1 + a] = (mask & (kTermSponge | (1u << (2 * a)))) != 0u;
+    const Rsrc ro = make_rsrc(ep_out + wofs, wb64);
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
-      if (touch[f]) {
-        const Rsrc ro = make_rsrc(ep_out + f * fs, fb64);
-        bstore<double>(bload<double>(ro, gi) + (double)rhs[f], ro, gi);
-      }
+      if (touch[f]) bstore<double>(bload<double>(ro, gi + f * ES) + (double)rhs[f], ro, gi + f * ES);
     }
   } else if constexpr (EPI == EPI_LSRK) {
+    const Rsrc rqo = make_rsrc(ep_qout + wofs, wb64);
     if constexpr (std::is_same<DUT, double>::value) {
       double dn[4];
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
         dn[f] = ep_a * pre[f] + ep_dt * (double)rhs[f];
-        if constexpr (!kDuPairMajor) bstore<double, kPolicyNT>(dn[f], rp[f], gi);
-        bstore<double, kPolicyNT>(q[f] + ep_b * dn[f], make_rsrc(ep_qout + f * fs, fb64), gi);
+        if constexpr (!kDuPairMajor) bstore<double, kPolicyNT>(dn[f], rdu, gi + f * ES);
+        bstore<double, kPolicyNT>(q[f] + ep_b * dn[f], rqo, gi + f * ES);
       }
       if constexpr (kDuPairMajor) {
         bstore2d<kPolicyNT>(dn[0], dn[1], rdu, du_off);
-        bstore2d<kPolicyNT>(dn[2], dn[3], rdu1, du_off);
+        bstore2d<kPolicyNT>(dn[2], dn[3], rdu, du_off + (uint32_t)ES * 16u);
       }
     } else {
       const float af = (float)ep_a, dtf = (float)ep_dt;
@@ -887,19 +879,20 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
         dn[f] = af * (float)pre[f] + dtf * (float)rhs[f];
-        if constexpr (!kDuNodeMajor) bstore<float, kPolicyNT>(dn[f], rp[f], gi);
-        bstore<double, kPolicyNT>(q[f] + ep_b * (double)dn[f], make_rsrc(ep_qout + f * fs, fb64), gi);
+        if constexpr (!kDuNodeMajor) bstore<float, kPolicyNT>(dn[f], rdu, gi + f * ES);
+        bstore<double, kPolicyNT>(q[f] + ep_b * (double)dn[f], rqo, gi + f * ES);
       }
       if constexpr (kDuNodeMajor) bstore4f<kPolicyNT>(dn, rdu, du_off);
     }
   } else {
     // Grid.cpp:705, 765-766, 826-827: out = (rhs*(dt*beta) + in*beta) + old*alpha
     const double sb = ep_dt * ep_b;
+    const Rsrc ro = make_rsrc(ep_out + wofs, wb64);
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
       double v = (double)rhs[f] * sb + q[f] * ep_b;
       if (ep_a != 0.0) v += pre[f] * ep_a;
-      bstore<double>(v, make_rsrc(ep_out + f * fs, fb64), gi);
+      bstore<double>(v, ro, gi + f * ES);
     }
   }
 }
@@ -1019,10 +1012,9 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
   const int node = tid - le * NP;
   const int e = e0 + le;
   const bool valid = (le < EPB) && (e < (int)m.E);
-  const int64_t gi = (int64_t)e * S::ES + node;
   const int axis = L.axis;
   const int fin = (L.which == 0) ? 0 : 4 + axis;  // rho' or Q_axis
-  if (valid) sV[le * NP + node] = L.in[fin * m.fs + gi];
+  if (valid) sV[le * NP + node] = L.in[reg_slot(m.fs, S::ES, fin, e, node)];
   __syncthreads();
   for (int f = tid; f < EPB * 2 * NF2; f += S::NT) {
     const int fle = f / (2 * NF2);
@@ -1041,7 +1033,7 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
     const int eo = neighbour(m, fe, ec, axis, side);
     const int dir = (axis == 0) ? 4 + side : (axis == 1 ? 2 + side : side);
     if (eo >= 0) {
-      vo = L.in[fin * m.fs + (int64_t)eo * S::ES + on];
+      vo = L.in[reg_slot(m.fs, S::ES, fin, eo, on)];
     } else {
       const int64_t fidx = face_elem(m, ec, axis);
       const double* tr = L.ext.trace[dir];
@@ -1069,9 +1061,9 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
   elem_coords(m, (uint32_t)e, ec);
   const double x = s * m.inv_w[axis][ec[axis]];
   if (L.which == 0) {
-    L.out[(4 + axis) * m.fs + gi] = -x;
+    L.out[reg_slot(m.fs, S::ES, 4 + axis, e, node)] = -x;
   } else {
-    L.out[gi] += -L.diff[e] * x;
+    L.out[reg_slot(m.fs, S::ES, 0, e, node)] += -L.diff[e] * x;
   }
 }
 
@@ -1097,10 +1089,10 @@ __global__ void __launch_bounds__((Geo<NN, Shape<NN>::EPB>::NT)) k_ldg_q(const L
   const int node = tid - le * ES;
   const int e = e0 + le;
   const bool valid = (le < EPB) && (e < E) && node < NP;
-  const int64_t gi = (int64_t)(valid ? e : 0) * ES + node;
+  const int64_t ve = valid ? e : 0;
 
   // ---- phase 0: every global load of the workgroup (as in the stage kernel) ---------------
-  const double v = valid ? L.in[gi] : 0.0;
+  const double v = valid ? L.in[ve * 4 * ES + node] : 0.0;
   const double* trace_ptr[6];
 #pragma unroll
   for (int d = 0; d < 6; ++d) trace_ptr[d] = L.ext.trace[d];
@@ -1133,7 +1125,7 @@ __global__ void __launch_bounds__((Geo<NN, Shape<NN>::EPB>::NT)) k_ldg_q(const L
         // the lane's own node and is zeroed below (kItemExterior)
         const bool inner = nb >= 0;
         const bool state = inner || tr == nullptr;
-        const int64_t oi = inner ? (int64_t)nb * ES + on : (int64_t)fe * ES + sn;
+        const int64_t oi = inner ? (int64_t)nb * 4 * ES + on : (int64_t)fe * 4 * ES + sn;
         const double* pv = state ? L.in + oi : tr + ((int64_t)(-1 - nb) * NF2 + fn) * 3;
         vo[r] = *pv;
         item[r] = (unsigned)fle | ((unsigned)dir << 6) | ((unsigned)fn << 9) | ((unsigned)sn << 16) |
@@ -1199,7 +1191,7 @@ __global__ void __launch_bounds__((Geo<NN, Shape<NN>::EPB>::NT)) k_ldg_q(const L
     for (int mm = 0; mm < NN; ++mm) sv += sA[ix * NN + mm] * sV[le * NP + base + mm * ls];
     sv += sFl[(le * 6 + dlo) * NF2 + fn] * sL0[ix] + sFl[(le * 6 + dlo + 1) * NF2 + fn] * sLN[ix];
     // nontemporal: the stage that reads Q next ran 3-4 % faster than after cached stores
-    __builtin_nontemporal_store(-(sv * sIw[le * 3 + a]), L.out + (4 + a) * m.fs + gi);
+    __builtin_nontemporal_store(-(sv * sIw[le * 3 + a]), L.out + 4 * m.fs + (ve * 3 + a) * ES + node);
   }
 }
 
@@ -1244,32 +1236,47 @@ __device__ double block_reduce(double v, Op op, double* sh) {
   return v;
 }
 
-__global__ void __launch_bounds__(kRedThreads) k_maxabs(const double* reg, int64_t fs, int nfields, double* part) {
+// The state fields' region of a register, slots 0 .. 4 fs - 1: slot i belongs to field
+// (i / es) mod 4 (past the last element: padding, zeros).
+__global__ void __launch_bounds__(kRedThreads) k_maxabs(const double* reg, int64_t fs, int es, double* part) {
   __shared__ double sh[kRedThreads / 64];
-  for (int f = 0; f < nfields; ++f) {
-    double v = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < fs; i += (int64_t)gridDim.x * blockDim.x)
-      v = nan_max(v, fabs(reg[f * fs + i]));
-    v = block_reduce(v, nan_max, sh);
-    if (threadIdx.x == 0) part[blockIdx.x * nfields + f] = v;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 4 * fs; i += (int64_t)gridDim.x * blockDim.x) {
+    const int f = (int)((uint32_t)i / (uint32_t)es) & 3;  // 4 fs < 2^31
+    const double x = fabs(reg[i]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k == f) v[k] = nan_max(v[k], x);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double r = block_reduce(v[k], nan_max, sh);
+    if (threadIdx.x == 0) part[blockIdx.x * 4 + k] = r;
   }
 }
 
-__global__ void __launch_bounds__(kRedThreads) k_l2diff(const double* a, const double* b, int64_t fs, int nfields, double* part) {
+__global__ void __launch_bounds__(kRedThreads) k_l2diff(const double* a, const double* b, int64_t fs, int es, double* part) {
   __shared__ double sh[kRedThreads / 64];
   auto add = [](double x, double y) { return x + y; };
-  for (int f = 0; f < nfields; ++f) {
-    double d = 0.0, r = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < fs; i += (int64_t)gridDim.x * blockDim.x) {
-      const double x = a[f * fs + i], y = b[f * fs + i];
-      d += (x - y) * (x - y);
-      r += y * y;
+  double d[4] = {0.0, 0.0, 0.0, 0.0}, r[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 4 * fs; i += (int64_t)gridDim.x * blockDim.x) {
+    const int f = (int)((uint32_t)i / (uint32_t)es) & 3;  // 4 fs < 2^31
+    const double x = a[i], y = b[i];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k == f) {
+        d[k] += (x - y) * (x - y);
+        r[k] += y * y;
+      }
     }
-    d = block_reduce(d, add, sh);
-    r = block_reduce(r, add, sh);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double dd = block_reduce(d[k], add, sh);
+    const double rr = block_reduce(r[k], add, sh);
     if (threadIdx.x == 0) {
-      part[(blockIdx.x * nfields + f) * 2 + 0] = d;
-      part[(blockIdx.x * nfields + f) * 2 + 1] = r;
+      part[(blockIdx.x * 4 + k) * 2 + 0] = dd;
+      part[(blockIdx.x * 4 + k) * 2 + 1] = rr;
     }
   }
 }
@@ -1280,7 +1287,7 @@ __global__ void __launch_bounds__(kRedThreads) k_energy(const double* reg, MeshA
   const int np = n * n * n;
   double en = 0.0, ms = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m.fs; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t e = i / m.es;
+    const int64_t e = i / m.es;  // node slot i of the block, element-major
     const int nd = (int)(i - e * m.es);
     if (e >= m.E || nd >= np) continue;  // padding slots
     const int a = nd / (n * n), b = (nd / n) % n, c = nd % n;
@@ -1289,8 +1296,9 @@ __global__ void __launch_bounds__(kRedThreads) k_energy(const double* reg, MeshA
     const double jac = 0.125 / (m.inv_w[0][ec[0]] * m.inv_w[1][ec[1]] * m.inv_w[2][ec[2]]);
     const double w = wq[a] * wq[b] * wq[c] * jac;
     const double c0 = m.mat[e * kMatStride + MAT_C], r0 = m.mat[e * kMatStride + MAT_RHO0];
-    const double r = reg[i];
-    const double u = reg[m.fs + i], v = reg[2 * m.fs + i], x = reg[3 * m.fs + i];
+    const double* blk = reg + e * 4 * m.es + nd;
+    const double r = blk[0];
+    const double u = blk[m.es], v = blk[2 * m.es], x = blk[3 * m.es];
     en += w * 0.5 * (c0 * c0 * r * r / r0 + r0 * (u * u + v * v + x * x));
     ms += w * r;
   }
@@ -1323,8 +1331,8 @@ __global__ void k_probe(const double* reg, MeshArgs m, int n, const int64_t* nod
   const int row = step[0];
   if (i >= count || row >= cap) return;
   double* out = buf + (int64_t)row * count;
-  const int64_t g = nodes[i];  // device slot
-  const int64_t e = g / m.es;
+  const int64_t g = nodes[i];  // device slot of the node's rho'
+  const int64_t e = g / (4 * m.es);
   const double c = m.mat[e * kMatStride + MAT_C], r0 = m.mat[e * kMatStride + MAT_RHO0];
   const double ba = m.mat[e * kMatStride + MAT_BA];
   const double r = reg[g];
@@ -1647,12 +1655,12 @@ int reduce_blocks(int64_t count) {
   return (int)b;
 }
 
-hipError_t launch_maxabs(const double* reg, int64_t fs, int nfields, double* part, hipStream_t s) {
-  hipLaunchKernelGGL(k_maxabs, dim3(reduce_blocks(fs)), dim3(kRedThreads), 0, s, reg, fs, nfields, part);
+hipError_t launch_maxabs(const double* reg, int64_t fs, int es, double* part, hipStream_t s) {
+  hipLaunchKernelGGL(k_maxabs, dim3(reduce_blocks(fs)), dim3(kRedThreads), 0, s, reg, fs, es, part);
   return hipGetLastError();
 }
-hipError_t launch_l2diff(const double* a, const double* b, int64_t fs, int nfields, double* part, hipStream_t s) {
-  hipLaunchKernelGGL(k_l2diff, dim3(reduce_blocks(fs)), dim3(kRedThreads), 0, s, a, b, fs, nfields, part);
+hipError_t launch_l2diff(const double* a, const double* b, int64_t fs, int es, double* part, hipStream_t s) {
+  hipLaunchKernelGGL(k_l2diff, dim3(reduce_blocks(fs)), dim3(kRedThreads), 0, s, a, b, fs, es, part);
   return hipGetLastError();
 }
 hipError_t launch_energy(const double* reg, const MeshArgs& m, int n, const double* wq, double* part, hipStream_t s) {
