@@ -404,6 +404,7 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
     m.E = h.E;
     m.fs = h.fs;
     m.es = h.es;
+    m.grp = dg3d::group_elems(h.n);
     m.stride[0] = (int64_t)h.nE[1] * h.nE[2];
     m.stride[1] = h.nE[2];
     m.stride[2] = 1;
@@ -496,7 +497,7 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
     stage.assign((size_t)4 * h.fs, 0.0);
     for (int f = 0; f < 4; ++f)
       for (int64_t el = 0; el < h.E; ++el)
-        std::memcpy(&stage[dg3d::reg_slot(h.fs, h.es, f, el, 0)], d.state + ((size_t)f * h.E + el) * h.np,
+        std::memcpy(&stage[dg3d::reg_slot(h.fs, h.es, dg3d::group_elems(h.n), f, el, 0)], d.state + ((size_t)f * h.E + el) * h.np,
                     sizeof(double) * h.np);
     e = hipMemcpy(b->arena[0] + h.slot0, stage.data(), sizeof(double) * stage.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(DG3D_E_HIP, std::string("batch: state upload: ") + hipGetErrorString(e));
@@ -648,7 +649,7 @@ int dg3d_batch_get_state(dg3d_batch* b, int solve, double* host) {
   if (e != hipSuccess) return b->hip_fail(e, "batch get_state");
   for (int f = 0; f < 4; ++f)
     for (int64_t el = 0; el < h.E; ++el)
-      std::memcpy(host + ((size_t)f * h.E + el) * h.np, &stage[dg3d::reg_slot(h.fs, h.es, f, el, 0)],
+      std::memcpy(host + ((size_t)f * h.E + el) * h.np, &stage[dg3d::reg_slot(h.fs, h.es, dg3d::group_elems(h.n), f, el, 0)],
                   sizeof(double) * h.np);
   return DG3D_OK;
 }

@@ -52,6 +52,7 @@ struct Register {
 
 struct dg3d_ctx {
   int N = 0, n = 0, np = 0, es = 0;  // es: node slots per element in device memory
+  int grp = 1;                        // elements per group of the register layout
   int nE[3] = {0, 0, 0};
   int periodic[3] = {0, 0, 0};
   int64_t E = 0, fs = 0;
@@ -129,6 +130,7 @@ struct dg3d_ctx {
     m.E = E;
     m.fs = fs;
     m.es = es;
+    m.grp = grp;
     m.stride[0] = (int64_t)nE[1] * nE[2];
     m.stride[1] = nE[2];
     m.stride[2] = 1;
@@ -266,19 +268,21 @@ int ensure_work(dg3d_ctx* c, int scheme) {
   return DG3D_OK;
 }
 
-// Fields 0 .. nfields-1 between the host layout ([field][E][n^3]) and a register's
-// element-blocked slots (dg3d::reg_slot), in chunks of elements: a chunk's state fields are one
-// contiguous run of the register, and so are its Q fields. Writing only some of a block's fields
-// reads the chunk first; the padding slots of fully written blocks are written as zeros.
+// Fields 0 .. nfields-1 between the host layout ([field][E][n^3]) and a register's group-blocked
+// slots (dg3d::reg_slot), in chunks of whole groups: a chunk's state fields are one contiguous
+// run of the register, and so are its Q fields. Writing only some of a block's fields reads the
+// chunk first; otherwise the gaps and padding slots are written as zeros.
 hipError_t transfer_state(const dg3d_ctx* c, double* dev, double* host, int nfields, bool to_device) {
+  constexpr int64_t P = dg3d::kRunSlots;
   const int es = c->es, np = c->np;
-  const int64_t chunk = std::max<int64_t>(1, ((int64_t)1 << 22) / (4 * es));
+  const int64_t G = c->grp, groups = (c->E + G - 1) / G;
+  const int64_t chunk = std::max<int64_t>(1, ((int64_t)1 << 22) / (4 * P));  // groups per chunk
   std::vector<double> buf;
   hipError_t e = hipSuccess;
-  // fields f_lo .. f_lo+f_n-1 held blk to an element block, the blocks starting at slot d_base
-  auto part = [&](double* d_base, int blk, int f_lo, int f_n, int64_t e0, int64_t e1) {
-    const size_t count = (size_t)(e1 - e0) * blk * es;
-    double* d = d_base + (size_t)e0 * blk * es;
+  // fields f_lo .. f_lo+f_n-1 held in blocks of blk runs from slot d_base, groups g0 .. g1-1
+  auto part = [&](double* d_base, int blk, int f_lo, int f_n, int64_t g0, int64_t g1) {
+    const size_t count = (size_t)(g1 - g0) * blk * P;
+    double* d = d_base + (size_t)g0 * blk * P;
     if (to_device && f_n == blk) {
       buf.assign(count, 0.0);
     } else {
@@ -286,9 +290,9 @@ hipError_t transfer_state(const dg3d_ctx* c, double* dev, double* host, int nfie
       e = hipMemcpy(buf.data(), d, sizeof(double) * count, hipMemcpyDeviceToHost);
       if (e != hipSuccess) return;
     }
-    for (int64_t el = e0; el < e1; ++el)
+    for (int64_t el = g0 * G; el < std::min(c->E, g1 * G); ++el)
       for (int k = 0; k < f_n; ++k) {
-        double* slot = &buf[((size_t)(el - e0) * blk + k) * es];
+        double* slot = &buf[(size_t)((el / G - g0) * blk + k) * P + (size_t)(el % G) * es];
         double* h = host + ((size_t)(f_lo + k) * c->E + el) * np;
         if (to_device) std::memcpy(slot, h, sizeof(double) * np);
         else std::memcpy(h, slot, sizeof(double) * np);
@@ -296,10 +300,10 @@ hipError_t transfer_state(const dg3d_ctx* c, double* dev, double* host, int nfie
     if (to_device) e = hipMemcpy(d, buf.data(), sizeof(double) * count, hipMemcpyHostToDevice);
   };
   const int sf = std::min(nfields, 4), qf = nfields - sf;
-  for (int64_t e0 = 0; e0 < c->E && e == hipSuccess; e0 += chunk) {
-    const int64_t e1 = std::min(c->E, e0 + chunk);
-    part(dev, 4, 0, sf, e0, e1);
-    if (e == hipSuccess && qf > 0) part(dev + 4 * c->fs, 3, 4, qf, e0, e1);
+  for (int64_t g0 = 0; g0 < groups && e == hipSuccess; g0 += chunk) {
+    const int64_t g1 = std::min(groups, g0 + chunk);
+    part(dev, 4, 0, sf, g0, g1);
+    if (e == hipSuccess && qf > 0) part(dev + 4 * c->fs, 3, 4, qf, g0, g1);
   }
   return e;
 }
@@ -675,6 +679,7 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
     }
     c->E = (int64_t)c->nE[0] * c->nE[1] * c->nE[2];
     c->es = dg3d::elem_stride(c->n);
+    c->grp = dg3d::group_elems(c->n);
     c->fs = dg3d::field_slots(c->E, c->n);
     c->nfields = d->with_q ? 7 : 4;
     c->precision = d->precision;
@@ -1071,9 +1076,9 @@ bool reduce_shape(int kind, int* per_block, int* combine) {
 }
 
 hipError_t launch_reduction(dg3d_ctx* c, int kind, int reg, int reg2, double* part) {
-  if (kind == DG3D_REDUCE_MAXABS) return dg3d::launch_maxabs(c->regs[reg].d, c->fs, c->es, part, c->stream);
+  if (kind == DG3D_REDUCE_MAXABS) return dg3d::launch_maxabs(c->regs[reg].d, c->fs, part, c->stream);
   if (kind == DG3D_REDUCE_ENERGY) return dg3d::launch_energy(c->regs[reg].d, c->mesh(), c->n, c->d_wq, part, c->stream);
-  return dg3d::launch_l2diff(c->regs[reg].d, c->regs[reg2].d, c->fs, c->es, part, c->stream);
+  return dg3d::launch_l2diff(c->regs[reg].d, c->regs[reg2].d, c->fs, part, c->stream);
 }
 
 // The outputs of a reduction kind from its combined values (v: 4 maxima, the 2 energy sums, or
@@ -1180,7 +1185,7 @@ int dg3d_set_probes(dg3d_ctx* c, int count, const int64_t* nodes, int capacity_s
   std::vector<int64_t> slots(count);
   for (int i = 0; i < count; ++i) {
     if (nodes[i] < 0 || nodes[i] >= c->E * c->np) return c->fail(DG3D_E_INVALID, "probe node out of range");
-    slots[i] = dg3d::reg_slot(c->fs, c->es, 0, nodes[i] / c->np, (int)(nodes[i] % c->np));  // host -> rho' slot
+    slots[i] = nodes[i];  // host numbering; the probe kernel finds the slot
   }
   if (dmalloc(&c->d_probe_nodes, count) != hipSuccess ||
       dmalloc(&c->d_probe_buf, (size_t)count * std::max(capacity_steps, 1)) != hipSuccess)

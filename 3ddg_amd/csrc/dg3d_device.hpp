@@ -129,28 +129,47 @@ __host__ __device__ inline uint32_t map_group(uint32_t w, const GroupMap& g) {
   return w;
 }
 
-// Device layout of a register: element-blocked. Each element has ES(n) node slots per field:
-// n^3 rounded up to a multiple of 32 slots (n = 3, 5, 6, 7), so every element's field starts on a
-// 128-byte line and a workgroup's elements cover whole lines: no line is written in part by two
-// workgroups. The padding slots hold zeros. n = 2 keeps 8 slots: its workgroups take 16 or 32
-// elements, which are whole lines too.
-//   * The state fields rho', u, v, w of element e are one block of 4 ES slots: field f at
-//     (4 e + f) ES + node. A workgroup's elements are then one contiguous run of the register,
-//     which it reads and writes as one stream: a probe of the LSRK stage's pattern (state in,
-//     RK register in place, state out; 64^3 N=4 FP64) streamed 76 % of the HBM peak in this
-//     layout, 72 % with every field its own array (and 68-69 % with the four fields of a node
-//     side by side, 32 bytes per lane).
-//   * The LDG fields Q_x, Q_y, Q_z of a 7-field register follow in blocks of 3 ES slots from
-//     slot 4 fs: field 4 + a at 4 fs + (3 e + a) ES + node.
-//   * A register takes nfields * fs slots, fs = round_up(E * ES, 2048) + 256 (the slots past the
-//     last element hold zeros).
+// Device layout of a register: group-blocked.
+//   * Each element has ES(n) node slots per field: n^3 rounded up to a multiple of 32 slots
+//     (n = 3, 5, 6, 7), so every element's field starts on a 128-byte line and a workgroup's
+//     elements cover whole lines: no line is written in part by two workgroups. The padding slots
+//     hold zeros. n = 2 keeps 8 slots: its workgroups take 16 or 32 elements, whole lines too.
+//   * Consecutive elements form groups of G(n) elements. A group's state fields rho', u, v, w are
+//     one block of 4 runs of kRunSlots slots: field f of element e at
+//     (e / G) 4 P + f P + (e % G) ES + node, P = kRunSlots. G ES <= P; the slots past a group's
+//     elements in each run are a gap that nothing reads. The LDG fields Q_x, Q_y, Q_z of a
+//     7-field register follow from slot 4 fs in blocks of 3 runs: field 4 + a at
+//     4 fs + (e / G) 3 P + a P + (e % G) ES + node.
+//   * Why runs of 4096 FP64 values: a probe of the LSRK stage's stream (state read, RK register
+//     read and written in place, state written; 64^3 N=4 FP64) ran at 83-84 % of the HBM peak
+//     with runs of 4096 slots at a spacing of 4096, at 79-83 % with shorter runs of 3520-4032
+//     slots at that spacing, but at 74-76 % with unpadded runs of 3584-6144 slots, 71-73 % with
+//     runs of one element (128 slots) or with every field its own array, and at 68-70 % whenever
+//     the spacing was not a power of two. Every workgroup's elements lie in one group (G is a
+//     multiple of every elements-per-workgroup shape of the order), so a workgroup streams one
+//     piece of each of the 4 runs of its group.
+//   * A register takes nfields * fs slots, fs = ceil(E / G) P (the gaps and the slots of a last,
+//     partial group hold zeros).
+constexpr int kRunSlots = 4096;
 constexpr int elem_stride(int n) {
   return (n == 2 || (n * n * n) % 32 == 0) ? n * n * n : (n * n * n + 31) / 32 * 32;
 }
-inline int64_t field_slots(int64_t E, int n) { return (E * elem_stride(n) + 2047) / 2048 * 2048 + 256; }
-// Slot of field f (0..6) of element e, node slot `node`, in a register of field unit fs.
-__host__ __device__ inline int64_t reg_slot(int64_t fs, int es, int f, int64_t e, int node) {
-  return f < 4 ? (e * 4 + f) * es + node : 4 * fs + (e * 3 + (f - 4)) * es + node;
+// elements per group: as many as fit a run, rounded down to a multiple of the order's
+// elements-per-workgroup shapes (16 and 32 at n = 2, 8 and 16 at n = 3, 4 and 8 at n = 4, 2 and 4
+// at n = 5, 2 and 1 at n = 6 and 7, 1 and 2 at n = 8)
+constexpr int group_elems(int n) {
+  return n == 2 ? 512 : n == 3 ? 128 : n == 4 ? 64 : n == 5 ? 32 : n == 6 ? 18 : n == 7 ? 10 : 8;
+}
+inline int64_t field_slots(int64_t E, int n) {
+  const int64_t g = group_elems(n);
+  return (E + g - 1) / g * kRunSlots;
+}
+// Slot of field f (0..6) of element e, node slot `node`, in a register of field unit fs with es
+// slots per element and g elements per group.
+__host__ __device__ inline int64_t reg_slot(int64_t fs, int es, int g, int f, int64_t e, int node) {
+  const int64_t grp = e / g, eg = e - grp * g;
+  return f < 4 ? grp * 4 * kRunSlots + (int64_t)f * kRunSlots + eg * es + node
+               : 4 * fs + grp * 3 * kRunSlots + (int64_t)(f - 4) * kRunSlots + eg * es + node;
 }
 
 constexpr int kMatStride = 8;  // per-element material record (doubles)
@@ -162,6 +181,7 @@ struct MeshArgs {
   int64_t E;          // interior elements (< 2^31)
   int64_t fs;         // field unit of a register: field_slots(E, n) (layout: reg_slot)
   int es;             // node slots per element: elem_stride(n)
+  int grp;            // elements per group: group_elems(n)
   int64_t stride[3];  // element index stride per axis: e = (i*nE1 + j)*nE2 + k
   Div32 div12;        // divide by nE1*nE2
   Div32 div2;         // divide by nE2
@@ -286,10 +306,10 @@ hipError_t launch_fill_pattern(double* out, double amp, int64_t count, hipStream
 
 // Reductions: partial results per block into `partials` (size returned by reduce_blocks). The
 // max |.| and L2 reductions cover every slot of the four state fields' region (4 fs slots, the
-// slots past the last element included) of a register with es slots per element.
+// gaps and the slots past the last element included): slot i belongs to field (i / P) mod 4.
 int reduce_blocks(int64_t count);
-hipError_t launch_maxabs(const double* reg, int64_t fs, int es, double* partials, hipStream_t s);
-hipError_t launch_l2diff(const double* a, const double* b, int64_t fs, int es, double* partials, hipStream_t s);
+hipError_t launch_maxabs(const double* reg, int64_t fs, double* partials, hipStream_t s);
+hipError_t launch_l2diff(const double* a, const double* b, int64_t fs, double* partials, hipStream_t s);
 hipError_t launch_energy(const double* reg, const MeshArgs& m, int n, const double* wq, double* partials,
                          hipStream_t s);
 hipError_t launch_cfl(const MeshArgs& m, double dxi_half, double* partials, hipStream_t s);
@@ -304,7 +324,8 @@ constexpr int kScratchPartials = 1024 * 8;
 constexpr int kScratchValues = 8;
 hipError_t scratch_address(int device, double** out);
 hipError_t launch_reduce_combine(const double* partials, int blocks, int kind, double* out, hipStream_t s);
-// Probe values into row *step of `out` ([cap][count]); nothing once *step reaches cap.
+// Probe values into row *step of `out` ([cap][count]); nothing once *step reaches cap. `nodes`
+// holds host node indices (element * n^3 + node).
 hipError_t launch_probe(const double* reg, const MeshArgs& m, int n, const int64_t* nodes, int count,
                         double* out, const int* step, int cap, hipStream_t s);
 // Device clock of dg3d_step: clock[0] = t; the step counter of the probes is left alone.

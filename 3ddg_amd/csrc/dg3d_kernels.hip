@@ -27,9 +27,26 @@ struct Geo {
   static constexpr int NP = NN * NN * NN;
   static constexpr int NF2 = NN * NN;
   static constexpr int EPB = EPB_;
-  static constexpr int ES = elem_stride(NN);  // node slots per element in a field array
+  static constexpr int ES = elem_stride(NN);  // node slots per element in a field run
   static constexpr int NT = ((EPB * ES + 63) / 64) * 64;
+  static constexpr int G = group_elems(NN);     // elements per group of the register layout
+  static_assert(G % EPB_ == 0 && G * ES <= kRunSlots, "a workgroup's elements lie in one group");
 };
+
+// Register slots of the group-blocked layout (dg3d_device.hpp): field 0 of element e (< 2^31) at
+// node slot `node`; field f is f * kRunSlots further on. The Q fields: field 4 + a of element e.
+template <int NN>
+__device__ __forceinline__ int64_t state_slot(uint32_t e, int node) {
+  constexpr uint32_t G = group_elems(NN);
+  const uint32_t g = e / G;
+  return (int64_t)g * 4 * kRunSlots + (int64_t)(e - g * G) * elem_stride(NN) + node;
+}
+template <int NN>
+__device__ __forceinline__ int64_t q_slot(int64_t fs, uint32_t e, int a, int node) {
+  constexpr uint32_t G = group_elems(NN);
+  const uint32_t g = e / G;
+  return 4 * fs + (int64_t)g * 3 * kRunSlots + (int64_t)a * kRunSlots + (int64_t)(e - g * G) * elem_stride(NN) + node;
+}
 // Default shape per order (whole waves with < 10% idle lanes); used by the LDG kernel.
 template <int NN>
 struct Shape : Geo<NN, (NN == 2) ? 32 : (NN == 3) ? 8 : (NN == 4) ? 4 : (NN == 5) ? 2 : (NN == 6) ? 2 : (NN == 7) ? 2 : 1> {};
@@ -497,32 +514,40 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   // lanes past the workgroup's elements load a slot of its last element, a line that its own
   // lanes read anyway, rather than one fixed element that every workgroup would read
   const int ge = io ? e : min(e0 + EPB - 1, E - 1);
-  // Every register access of the workgroup goes through a buffer resource over its own elements'
-  // blocks (elements e0 .. e0+ne-1, contiguous in the element-blocked layout), so the offsets
-  // are small whatever the register's size. gi: the lane's field-0 slot in that run; field f is
-  // ES slots further on.
+  // Every register access of the workgroup goes through buffer resources that start at its first
+  // element's slot in field 0 and span its elements' pieces of the 4 runs of their group (the
+  // workgroup lies in one group), so the offsets are small whatever the register's size. gi: the
+  // lane's slot from there; field f is P = kRunSlots slots further on.
+  constexpr int P = kRunSlots;
   const int ne = (E - e0 < EPB) ? E - e0 : EPB;
-  const uint32_t gi = (uint32_t)((ge - e0) * 4 * ES + node);
-  const uint32_t wb64 = (uint32_t)(ne * 4 * ES * 8);
-  const size_t wofs = (size_t)e0 * 4 * ES;  // the run's first slot
+  const uint32_t gi = (uint32_t)((ge - e0) * ES + node);
+  const uint32_t wb64 = (uint32_t)((3 * P + ne * ES) * 8);
+  const size_t wofs = (size_t)state_slot<NN>((uint32_t)e0, 0);
   const Rsrc rq = make_rsrc(L.q_in + wofs, wb64);
   double q[4], pre[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int f = 0; f < 4; ++f) q[f] = bload<double>(rq, gi + f * ES);
-  // The LSRK register is never read across a face, so its layout within an element's block is
-  // free. A lane can move its node's share in 16-byte accesses: the FP64 register pair-major
-  // (an element's (rho', u) pairs, then its (v, w) pairs: two accesses), the FP32 register
-  // node-major (four fields in one access). Otherwise it is laid out like the state. Measured per
-  // shape: at 7 nodes per line both 16-byte forms ran 4-6 % slower; at every other order they
-  // are faster or even
+  for (int f = 0; f < 4; ++f) q[f] = bload<double>(rq, gi + f * P);
+  // The LSRK register is never read across a face, so its layout within a group's block of 4 P
+  // values is free. A lane can move its node's share in 16-byte accesses: the FP64 register
+  // pair-major (the group's (rho', u) pairs in the block's first half, its (v, w) pairs in the
+  // second: two accesses), the FP32 register node-major (four fields in one access). Otherwise it
+  // is laid out like the state.
   constexpr bool kDuNodeMajor = NN != 7 && std::is_same<DUT, float>::value;
   constexpr bool kDuPairMajor = NN != 7 && std::is_same<DUT, double>::value;
   Rsrc rdu;
   uint32_t du_off = 0;
   if constexpr (EPI == EPI_LSRK) {
-    rdu = make_rsrc(static_cast<const DUT*>(L.du) + wofs, (uint32_t)(ne * 4 * ES * sizeof(DUT)));
-    if constexpr (kDuNodeMajor) du_off = (uint32_t)((ge - e0) * ES + node) * 16u;
-    if constexpr (kDuPairMajor) du_off = (uint32_t)((ge - e0) * 2 * ES + node) * 16u;  // pair 1: + ES * 16
+    const DUT* d0 = static_cast<const DUT*>(L.du) + (size_t)(e0 / S::G) * 4 * P;  // the group's block
+    const int eg = e0 % S::G;
+    if constexpr (kDuNodeMajor) {
+      rdu = make_rsrc(d0 + (size_t)eg * ES * 4, (uint32_t)(ne * ES * 16));
+      du_off = gi * 16u;
+    } else if constexpr (kDuPairMajor) {
+      rdu = make_rsrc(d0 + (size_t)eg * ES * 2, (uint32_t)((2 * P + ne * ES * 2) * 8));
+      du_off = gi * 16u;  // pair 1: + 2 P values
+    } else {
+      rdu = make_rsrc(d0 + (size_t)eg * ES, (uint32_t)((3 * P + ne * ES) * sizeof(DUT)));
+    }
   }
   auto load_pre = [&]() {
     if constexpr (EPI == EPI_LSRK) {
@@ -531,10 +556,10 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
           bload4f<kPolicyNT>(rdu, du_off, pre);
         } else if constexpr (kDuPairMajor) {
           bload2d<kPolicyNT>(rdu, du_off, pre[0], pre[1]);
-          bload2d<kPolicyNT>(rdu, du_off + (uint32_t)ES * 16u, pre[2], pre[3]);
+          bload2d<kPolicyNT>(rdu, du_off + (uint32_t)P * 16u, pre[2], pre[3]);
         } else {
 #pragma unroll
-          for (int f = 0; f < 4; ++f) pre[f] = (double)bload<DUT, kPolicyNT>(rdu, gi + f * ES);
+          for (int f = 0; f < 4; ++f) pre[f] = (double)bload<DUT, kPolicyNT>(rdu, gi + f * P);
         }
       }
     }
@@ -542,7 +567,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
       if (L.a != 0.0) {
         const Rsrc ro = make_rsrc(L.old + wofs, wb64);
 #pragma unroll
-        for (int f = 0; f < 4; ++f) pre[f] = bload<double, kPolicyNT>(ro, gi + f * ES);
+        for (int f = 0; f < 4; ++f) pre[f] = bload<double, kPolicyNT>(ro, gi + f * P);
       }
     }
   };
@@ -573,11 +598,10 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   double qq[3] = {0.0, 0.0, 0.0};
   CT dcoef = CT(0);
   if constexpr (LDG) {
-    // the Q fields: blocks of 3 ES slots from slot 4 fs
-    const Rsrc rqq = make_rsrc(L.q_in + 4 * fs + (size_t)e0 * 3 * ES, (uint32_t)(ne * 3 * ES * 8));
-    const uint32_t qi = (uint32_t)((ge - e0) * 3 * ES + node);
+    // the Q fields: blocks of 3 runs from slot 4 fs
+    const Rsrc rqq = make_rsrc(L.q_in + q_slot<NN>(fs, (uint32_t)e0, 0, 0), (uint32_t)((2 * P + ne * ES) * 8));
 #pragma unroll
-    for (int a = 0; a < 3; ++a) qq[a] = bload<double>(rqq, qi + a * ES);
+    for (int a = 0; a < 3; ++a) qq[a] = bload<double>(rqq, gi + a * P);
     dcoef = (CT)mat[MAT_DIFF];
   }
 
@@ -673,16 +697,16 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
         const bool local = inner && nb >= e0 && nb < e0 + EPB;
         item[r] |= local ? (kItemLocal | ((unsigned)(nb - e0) << 25)) : 0u;
         if (!local) {
-          const int64_t oe = inner ? nb : he;  // the element whose node is read
+          const uint32_t oe = (uint32_t)(inner ? nb : he);  // the element whose node is read
           const int onode = inner ? on : sn;
-          const int64_t oi = oe * 4 * ES + onode;
+          const int64_t oi = state_slot<NN>(oe, onode);
           const int64_t ti = ((int64_t)(-1 - nb) * NF2 + fn) * 3;
           const double* prho = state ? L.q_in + oi : tr + ti;
-          const double* pu = state ? L.q_in + oi + (1 + axis) * ES : tr + ti + 1;
+          const double* pu = state ? L.q_in + oi + (1 + axis) * P : tr + ti + 1;
           nrho[r] = *prho;
           nu[r] = *pu;
           if constexpr (LDG) {
-            const double* pq = state ? L.q_in + 4 * fs + (oe * 3 + axis) * ES + onode : tr + ti + 2;
+            const double* pq = state ? L.q_in + q_slot<NN>(fs, oe, axis, onode) : tr + ti + 2;
             nq[r] = *pq;
           }
         }
@@ -857,7 +881,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     const Rsrc ro = make_rsrc(ep_out + wofs, wb64);
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
-      if (touch[f]) bstore<double>(bload<double>(ro, gi + f * ES) + (double)rhs[f], ro, gi + f * ES);
+      if (touch[f]) bstore<double>(bload<double>(ro, gi + f * P) + (double)rhs[f], ro, gi + f * P);
     }
   } else if constexpr (EPI == EPI_LSRK) {
     const Rsrc rqo = make_rsrc(ep_qout + wofs, wb64);
@@ -866,12 +890,12 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
         dn[f] = ep_a * pre[f] + ep_dt * (double)rhs[f];
-        if constexpr (!kDuPairMajor) bstore<double, kPolicyNT>(dn[f], rdu, gi + f * ES);
-        bstore<double, kPolicyNT>(q[f] + ep_b * dn[f], rqo, gi + f * ES);
+        if constexpr (!kDuPairMajor) bstore<double, kPolicyNT>(dn[f], rdu, gi + f * P);
+        bstore<double, kPolicyNT>(q[f] + ep_b * dn[f], rqo, gi + f * P);
       }
       if constexpr (kDuPairMajor) {
         bstore2d<kPolicyNT>(dn[0], dn[1], rdu, du_off);
-        bstore2d<kPolicyNT>(dn[2], dn[3], rdu, du_off + (uint32_t)ES * 16u);
+        bstore2d<kPolicyNT>(dn[2], dn[3], rdu, du_off + (uint32_t)P * 16u);
       }
     } else {
       const float af = (float)ep_a, dtf = (float)ep_dt;
@@ -879,8 +903,8 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
         dn[f] = af * (float)pre[f] + dtf * (float)rhs[f];
-        if constexpr (!kDuNodeMajor) bstore<float, kPolicyNT>(dn[f], rdu, gi + f * ES);
-        bstore<double, kPolicyNT>(q[f] + ep_b * (double)dn[f], rqo, gi + f * ES);
+        if constexpr (!kDuNodeMajor) bstore<float, kPolicyNT>(dn[f], rdu, gi + f * P);
+        bstore<double, kPolicyNT>(q[f] + ep_b * (double)dn[f], rqo, gi + f * P);
       }
       if constexpr (kDuNodeMajor) bstore4f<kPolicyNT>(dn, rdu, du_off);
     }
@@ -892,7 +916,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     for (int f = 0; f < 4; ++f) {
       double v = (double)rhs[f] * sb + q[f] * ep_b;
       if (ep_a != 0.0) v += pre[f] * ep_a;
-      bstore<double>(v, ro, gi + f * ES);
+      bstore<double>(v, ro, gi + f * P);
     }
   }
 }
@@ -1014,7 +1038,7 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
   const bool valid = (le < EPB) && (e < (int)m.E);
   const int axis = L.axis;
   const int fin = (L.which == 0) ? 0 : 4 + axis;  // rho' or Q_axis
-  if (valid) sV[le * NP + node] = L.in[reg_slot(m.fs, S::ES, fin, e, node)];
+  if (valid) sV[le * NP + node] = L.in[reg_slot(m.fs, S::ES, group_elems(NN), fin, e, node)];
   __syncthreads();
   for (int f = tid; f < EPB * 2 * NF2; f += S::NT) {
     const int fle = f / (2 * NF2);
@@ -1033,7 +1057,7 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
     const int eo = neighbour(m, fe, ec, axis, side);
     const int dir = (axis == 0) ? 4 + side : (axis == 1 ? 2 + side : side);
     if (eo >= 0) {
-      vo = L.in[reg_slot(m.fs, S::ES, fin, eo, on)];
+      vo = L.in[reg_slot(m.fs, S::ES, group_elems(NN), fin, eo, on)];
     } else {
       const int64_t fidx = face_elem(m, ec, axis);
       const double* tr = L.ext.trace[dir];
@@ -1061,9 +1085,9 @@ __global__ void __launch_bounds__(Shape<NN>::NT) k_ldg(const LdgLaunch L) {
   elem_coords(m, (uint32_t)e, ec);
   const double x = s * m.inv_w[axis][ec[axis]];
   if (L.which == 0) {
-    L.out[reg_slot(m.fs, S::ES, 4 + axis, e, node)] = -x;
+    L.out[reg_slot(m.fs, S::ES, group_elems(NN), 4 + axis, e, node)] = -x;
   } else {
-    L.out[reg_slot(m.fs, S::ES, 0, e, node)] += -L.diff[e] * x;
+    L.out[reg_slot(m.fs, S::ES, group_elems(NN), 0, e, node)] += -L.diff[e] * x;
   }
 }
 
@@ -1092,7 +1116,7 @@ __global__ void __launch_bounds__((Geo<NN, Shape<NN>::EPB>::NT)) k_ldg_q(const L
   const int64_t ve = valid ? e : 0;
 
   // ---- phase 0: every global load of the workgroup (as in the stage kernel) ---------------
-  const double v = valid ? L.in[ve * 4 * ES + node] : 0.0;
+  const double v = valid ? L.in[state_slot<NN>((uint32_t)ve, node)] : 0.0;
   const double* trace_ptr[6];
 #pragma unroll
   for (int d = 0; d < 6; ++d) trace_ptr[d] = L.ext.trace[d];
@@ -1125,7 +1149,7 @@ __global__ void __launch_bounds__((Geo<NN, Shape<NN>::EPB>::NT)) k_ldg_q(const L
         // the lane's own node and is zeroed below (kItemExterior)
         const bool inner = nb >= 0;
         const bool state = inner || tr == nullptr;
-        const int64_t oi = inner ? (int64_t)nb * 4 * ES + on : (int64_t)fe * 4 * ES + sn;
+        const int64_t oi = state_slot<NN>((uint32_t)(inner ? nb : fe), inner ? on : sn);
         const double* pv = state ? L.in + oi : tr + ((int64_t)(-1 - nb) * NF2 + fn) * 3;
         vo[r] = *pv;
         item[r] = (unsigned)fle | ((unsigned)dir << 6) | ((unsigned)fn << 9) | ((unsigned)sn << 16) |
@@ -1191,7 +1215,7 @@ __global__ void __launch_bounds__((Geo<NN, Shape<NN>::EPB>::NT)) k_ldg_q(const L
     for (int mm = 0; mm < NN; ++mm) sv += sA[ix * NN + mm] * sV[le * NP + base + mm * ls];
     sv += sFl[(le * 6 + dlo) * NF2 + fn] * sL0[ix] + sFl[(le * 6 + dlo + 1) * NF2 + fn] * sLN[ix];
     // nontemporal: the stage that reads Q next ran 3-4 % faster than after cached stores
-    __builtin_nontemporal_store(-(sv * sIw[le * 3 + a]), L.out + 4 * m.fs + (ve * 3 + a) * ES + node);
+    __builtin_nontemporal_store(-(sv * sIw[le * 3 + a]), L.out + q_slot<NN>(m.fs, (uint32_t)ve, a, node));
   }
 }
 
@@ -1237,12 +1261,12 @@ __device__ double block_reduce(double v, Op op, double* sh) {
 }
 
 // The state fields' region of a register, slots 0 .. 4 fs - 1: slot i belongs to field
-// (i / es) mod 4 (past the last element: padding, zeros).
-__global__ void __launch_bounds__(kRedThreads) k_maxabs(const double* reg, int64_t fs, int es, double* part) {
+// (i / kRunSlots) mod 4 (gaps, padding and the slots past the last element hold zeros).
+__global__ void __launch_bounds__(kRedThreads) k_maxabs(const double* reg, int64_t fs, double* part) {
   __shared__ double sh[kRedThreads / 64];
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 4 * fs; i += (int64_t)gridDim.x * blockDim.x) {
-    const int f = (int)((uint32_t)i / (uint32_t)es) & 3;  // 4 fs < 2^31
+    const int f = (int)((uint32_t)i / (uint32_t)kRunSlots) & 3;  // 4 fs < 2^31
     const double x = fabs(reg[i]);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -1255,12 +1279,12 @@ __global__ void __launch_bounds__(kRedThreads) k_maxabs(const double* reg, int64
   }
 }
 
-__global__ void __launch_bounds__(kRedThreads) k_l2diff(const double* a, const double* b, int64_t fs, int es, double* part) {
+__global__ void __launch_bounds__(kRedThreads) k_l2diff(const double* a, const double* b, int64_t fs, double* part) {
   __shared__ double sh[kRedThreads / 64];
   auto add = [](double x, double y) { return x + y; };
   double d[4] = {0.0, 0.0, 0.0, 0.0}, r[4] = {0.0, 0.0, 0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 4 * fs; i += (int64_t)gridDim.x * blockDim.x) {
-    const int f = (int)((uint32_t)i / (uint32_t)es) & 3;  // 4 fs < 2^31
+    const int f = (int)((uint32_t)i / (uint32_t)kRunSlots) & 3;  // 4 fs < 2^31
     const double x = a[i], y = b[i];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -1296,9 +1320,9 @@ __global__ void __launch_bounds__(kRedThreads) k_energy(const double* reg, MeshA
     const double jac = 0.125 / (m.inv_w[0][ec[0]] * m.inv_w[1][ec[1]] * m.inv_w[2][ec[2]]);
     const double w = wq[a] * wq[b] * wq[c] * jac;
     const double c0 = m.mat[e * kMatStride + MAT_C], r0 = m.mat[e * kMatStride + MAT_RHO0];
-    const double* blk = reg + e * 4 * m.es + nd;
+    const double* blk = reg + reg_slot(m.fs, m.es, m.grp, 0, e, nd);
     const double r = blk[0];
-    const double u = blk[m.es], v = blk[2 * m.es], x = blk[3 * m.es];
+    const double u = blk[kRunSlots], v = blk[2 * kRunSlots], x = blk[3 * kRunSlots];
     en += w * 0.5 * (c0 * c0 * r * r / r0 + r0 * (u * u + v * v + x * x));
     ms += w * r;
   }
@@ -1331,11 +1355,12 @@ __global__ void k_probe(const double* reg, MeshArgs m, int n, const int64_t* nod
   const int row = step[0];
   if (i >= count || row >= cap) return;
   double* out = buf + (int64_t)row * count;
-  const int64_t g = nodes[i];  // device slot of the node's rho'
-  const int64_t e = g / (4 * m.es);
+  const int64_t g = nodes[i];  // host node index
+  const int64_t np = (int64_t)n * n * n;
+  const int64_t e = g / np;
   const double c = m.mat[e * kMatStride + MAT_C], r0 = m.mat[e * kMatStride + MAT_RHO0];
   const double ba = m.mat[e * kMatStride + MAT_BA];
-  const double r = reg[g];
+  const double r = reg[reg_slot(m.fs, m.es, m.grp, 0, e, (int)(g - e * np))];
   out[i] = c * c * (r + ba * 0.5 / r0 * r * r);
 }
 
@@ -1655,12 +1680,12 @@ int reduce_blocks(int64_t count) {
   return (int)b;
 }
 
-hipError_t launch_maxabs(const double* reg, int64_t fs, int es, double* part, hipStream_t s) {
-  hipLaunchKernelGGL(k_maxabs, dim3(reduce_blocks(fs)), dim3(kRedThreads), 0, s, reg, fs, es, part);
+hipError_t launch_maxabs(const double* reg, int64_t fs, double* part, hipStream_t s) {
+  hipLaunchKernelGGL(k_maxabs, dim3(reduce_blocks(fs)), dim3(kRedThreads), 0, s, reg, fs, part);
   return hipGetLastError();
 }
-hipError_t launch_l2diff(const double* a, const double* b, int64_t fs, int es, double* part, hipStream_t s) {
-  hipLaunchKernelGGL(k_l2diff, dim3(reduce_blocks(fs)), dim3(kRedThreads), 0, s, a, b, fs, es, part);
+hipError_t launch_l2diff(const double* a, const double* b, int64_t fs, double* part, hipStream_t s) {
+  hipLaunchKernelGGL(k_l2diff, dim3(reduce_blocks(fs)), dim3(kRedThreads), 0, s, a, b, fs, part);
   return hipGetLastError();
 }
 hipError_t launch_energy(const double* reg, const MeshArgs& m, int n, const double* wq, double* part, hipStream_t s) {

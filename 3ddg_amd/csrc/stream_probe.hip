@@ -1,31 +1,45 @@
-// Streaming ceiling of the device for the stage kernel's access pattern: 8 FP64 arrays of
-// 32 768 000 values read and 8 written, nontemporal, each 256-lane workgroup streaming one chunk
-// of consecutive values of every array, as a stage kernel's workgroups stream their elements'
-// fields (the 64^3 N=4 block has 32 768 000 nodes). No arithmetic beyond one multiply per
-// value, so the time is the memory system's. Prints one JSON line: the best rate over the
-// chunk sizes, counting the bytes read and written.
+// Streaming ceiling of the device for the LSRK stage kernel's access pattern, with no arithmetic
+// beyond one multiply-add per value, so the time is the memory system's. Three element-blocked
+// registers of the 64^3 N=4 block (32 768 000 nodes, 4 FP64 fields in blocks of 4 x 128 slots
+// per element): each workgroup takes a run of elements, reads their state blocks (cached loads)
+// and their RK-register blocks (pair-major, 16-byte nontemporal loads), and writes the
+// RK register in place and the new state (nontemporal), as a stage's workgroups do. Prints one
+// JSON line: the best rate over the workgroup sizes, counting 128 bytes per node (4 fields read
+// and written twice).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 
 namespace {
 
-constexpr int kArrays = 8;
-constexpr long kValues = 32768000;
-// array pitch in values: the stage's field pitch rule, 256 past a multiple of 2048 values (2 KiB
-// past a multiple of 16 KiB); pitches that are multiples of 1024 values stream 4-12 % slower
-constexpr long kStride = (kValues + 2047) / 2048 * 2048 + 256;
+constexpr long kNodes = 32768000;  // 262 144 elements of 125 nodes in 128 slots
+constexpr int kES = 128;
+constexpr long kSlots = kNodes / 125 * kES;  // per field
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
-__global__ void __launch_bounds__(256) k_stream(const double* __restrict__ in, double* __restrict__ out, int chunk) {
+// one lane per node slot; workgroup w owns elements [w * EPB, (w + 1) * EPB)
+template <int EPB>
+__global__ void __launch_bounds__(EPB * kES) k_stream(const double* __restrict__ qi, double* __restrict__ du,
+                                                    double* __restrict__ qo, double a, double b) {
   const int t = threadIdx.x;
-  if (t >= chunk) return;
-  const long i = (long)blockIdx.x * chunk + t;
-  if (i >= kValues) return;
-  double v[kArrays];
+  const int le = t / kES, node = t - le * kES;
+  const long e = (long)blockIdx.x * EPB + le;
+  const long s = e * 4 * kES + node;  // field 0 slot of the state block
+  double q[4];
 #pragma unroll
-  for (int a = 0; a < kArrays; ++a) v[a] = __builtin_nontemporal_load(in + a * kStride + i);
-#pragma unroll
-  for (int a = 0; a < kArrays; ++a) __builtin_nontemporal_store(v[a] * 1.0000001, out + a * kStride + i);
+  for (int f = 0; f < 4; ++f) q[f] = qi[s + f * kES];
+  f64x2* d = reinterpret_cast<f64x2*>(du + e * 4 * kES) + node;  // pair 0; pair 1 at + kES
+  f64x2 d0 = __builtin_nontemporal_load(d), d1 = __builtin_nontemporal_load(d + kES);
+  d0[0] = a * d0[0] + 1e-9 * q[0];
+  d0[1] = a * d0[1] + 1e-9 * q[1];
+  d1[0] = a * d1[0] + 1e-9 * q[2];
+  d1[1] = a * d1[1] + 1e-9 * q[3];
+  __builtin_nontemporal_store(d0, d);
+  __builtin_nontemporal_store(d1, d + kES);
+  __builtin_nontemporal_store(q[0] + b * d0[0], qo + s);
+  __builtin_nontemporal_store(q[1] + b * d0[1], qo + s + kES);
+  __builtin_nontemporal_store(q[2] + b * d1[0], qo + s + 2 * kES);
+  __builtin_nontemporal_store(q[3] + b * d1[1], qo + s + 3 * kES);
 }
 
 bool ok(hipError_t e, const char* what) {
@@ -34,36 +48,54 @@ bool ok(hipError_t e, const char* what) {
   return false;
 }
 
+template <int EPB>
+bool run(double* r0, double* r1, double* du, hipEvent_t e0, hipEvent_t e1, double* gbs) {
+  const int blocks = (int)(kSlots / kES / EPB);  // 262 144 elements: a multiple of EPB
+  auto launch = [&](int i) {
+    hipLaunchKernelGGL(k_stream<EPB>, dim3(blocks), dim3(EPB * kES), 0, 0, (i & 1) ? r1 : r0, du, (i & 1) ? r0 : r1,
+                       0.5, 0.25);
+  };
+  for (int w = 0; w < 4; ++w) launch(w);
+  if (!ok(hipGetLastError(), "launch") || !ok(hipEventRecord(e0, 0), "record")) return false;
+  const int reps = 40;
+  for (int r = 0; r < reps; ++r) launch(r);
+  if (!ok(hipEventRecord(e1, 0), "record") || !ok(hipEventSynchronize(e1), "sync")) return false;
+  float ms = 0.f;
+  if (!ok(hipEventElapsedTime(&ms, e0, e1), "elapsed")) return false;
+  *gbs = 128.0 * kNodes / (ms / reps * 1e-3) / 1e9;
+  return true;
+}
+
 }  // namespace
 
 int main() {
-  const size_t bytes = sizeof(double) * kArrays * kStride;
-  double *in = nullptr, *out = nullptr;
-  if (!ok(hipMalloc(&in, bytes), "hipMalloc") || !ok(hipMalloc(&out, bytes), "hipMalloc")) return 1;
-  if (!ok(hipMemset(in, 0, bytes), "hipMemset") || !ok(hipMemset(out, 0, bytes), "hipMemset")) return 1;
+  const size_t bytes = sizeof(double) * 4 * kSlots;
+  double *r0 = nullptr, *r1 = nullptr, *du = nullptr;
+  if (!ok(hipMalloc(&r0, bytes), "hipMalloc") || !ok(hipMalloc(&r1, bytes), "hipMalloc") ||
+      !ok(hipMalloc(&du, bytes), "hipMalloc"))
+    return 1;
+  if (!ok(hipMemset(r0, 0, bytes), "hipMemset") || !ok(hipMemset(r1, 0, bytes), "hipMemset") ||
+      !ok(hipMemset(du, 0, bytes), "hipMemset"))
+    return 1;
   hipEvent_t e0, e1;
   if (!ok(hipEventCreate(&e0), "event") || !ok(hipEventCreate(&e1), "event")) return 1;
-  const double moved = 2.0 * kArrays * kValues * sizeof(double);
   double best = 0.0;
-  int best_chunk = 0;
-  for (int chunk : {256, 250, 128, 125}) {
-    const int blocks = (int)((kValues + chunk - 1) / chunk);
-    for (int w = 0; w < 3; ++w) hipLaunchKernelGGL(k_stream, dim3(blocks), dim3(256), 0, 0, in, out, chunk);
-    if (!ok(hipGetLastError(), "launch") || !ok(hipEventRecord(e0, 0), "record")) return 1;
-    const int reps = 20;
-    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_stream, dim3(blocks), dim3(256), 0, 0, in, out, chunk);
-    if (!ok(hipEventRecord(e1, 0), "record") || !ok(hipEventSynchronize(e1), "sync")) return 1;
-    float ms = 0.f;
-    if (!ok(hipEventElapsedTime(&ms, e0, e1), "elapsed")) return 1;
-    const double gbs = moved / (ms / reps * 1e-3) / 1e9;
-    if (gbs > best) {
-      best = gbs;
-      best_chunk = chunk;
-    }
+  int best_lanes = 0;
+  for (int round = 0; round < 2; ++round) {
+    double g = 0.0;
+    if (!run<2>(r0, r1, du, e0, e1, &g)) return 1;
+    if (g > best) { best = g; best_lanes = 256; }
+    if (!run<4>(r0, r1, du, e0, e1, &g)) return 1;
+    if (g > best) { best = g; best_lanes = 512; }
+    if (!run<8>(r0, r1, du, e0, e1, &g)) return 1;
+    if (g > best) { best = g; best_lanes = 1024; }
   }
-  std::printf("{\"stream_GBs\": %.1f, \"chunk\": %d, \"arrays_read\": %d, \"arrays_written\": %d, \"values\": %ld}\n",
-              best, best_chunk, kArrays, kArrays, kValues);
-  (void)hipFree(in);
-  (void)hipFree(out);
+  std::printf("{\"stream_GBs\": %.1f, \"lanes\": %d, \"nodes\": %ld, \"pattern\": \"LSRK stage pattern: element-blocked "
+              "FP64 state read and written (4 fields per element block of 4 x 128 slots), pair-major RK register "
+              "read and written in place, %d-lane workgroups over whole elements, 128 B per node\"}\n",
+              best, best_lanes, kNodes, best_lanes);
+  (void)hipFree(r0);
+  (void)hipFree(r1);
+  (void)hipFree(du);
   return 0;
 }

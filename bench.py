@@ -90,10 +90,10 @@ STREAM_PROBE = os.path.join(ROOT, "3ddg_amd", "dg3d_stream_probe")
 
 
 def stream_ceiling():
-    """The device's streaming rate for the stage kernel's access pattern (8 FP64 arrays of the
-    64^3 N=4 block's node count read and 8 written, nontemporal, by 256-lane workgroups each
-    streaming one chunk of every array), from 3ddg_amd/dg3d_stream_probe run as a child process.
-    None when the probe is not built or fails."""
+    """The device's streaming rate for the LSRK stage kernel's access pattern at the 64^3 N=4
+    block's size (element-blocked FP64 state read and written, pair-major RK register read and
+    written in place, workgroups over whole elements), from 3ddg_amd/dg3d_stream_probe run as a
+    child process. None when the probe is not built or fails."""
     if not os.path.exists(STREAM_PROBE):
         return None
     try:
@@ -606,8 +606,7 @@ def main():
             out["stream_ceiling"] = dict(
                 GBs=sc["stream_GBs"], frac_of_peak=sc["stream_GBs"] / HBM_PEAK_GBS,
                 stage_frac_of_ceiling=achieved / sc["stream_GBs"],
-                pattern=f"{sc['arrays_read']} FP64 arrays read and {sc['arrays_written']} written, nontemporal, "
-                        f"256-lane workgroups streaming {sc['chunk']}-value chunks")
+                pattern=sc["pattern"])
         progress("streaming probe done")
         # the precision comparisons run from t = 0 until the whole burst has passed the probe
         # (at least the workloads' 500 steps), whatever --steps is

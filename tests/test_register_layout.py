@@ -1,24 +1,27 @@
 """The register addressing of the stage kernel (stage_body in 3ddg_amd/csrc/dg3d_kernels.hip),
 emulated on the CPU for every stage shape, both precisions and partial last workgroups.
 
-Registers are element-blocked: the four state fields of element e are one block of 4 ES slots
-(field f at (4 e + f) ES + node), and the LDG fields Q_x, Q_y, Q_z of a 7-field register follow
-from slot 4 fs in blocks of 3 ES slots. A workgroup reaches its elements' blocks through buffer
-resources that start at its first element's block and span its own elements only.
+Registers are group-blocked: G consecutive elements form a group whose state fields are 4 runs
+of P = 4096 slots, field f of element e at (e / G) 4P + f P + (e % G) ES + node; the LDG fields
+Q_x, Q_y, Q_z of a 7-field register follow from slot 4 fs in blocks of 3 runs. G ES <= P, and G is
+a multiple of every elements-per-workgroup shape, so a workgroup's elements lie in one group. A
+workgroup reaches them through buffer resources that start at its first element's slot.
 
-The LSRK register is never read across a face, so its layout within an element's block is the
-kernel's choice: pair-major FP64 (the element's (rho', u) pairs, then its (v, w) pairs) and
-node-major FP32 (the four fields of a node together) at every order except 7 nodes per line,
-where it is laid out like the state.
+The LSRK register is never read across a face, so its layout inside a group's block of 4P values
+is the kernel's choice: pair-major FP64 (the group's (rho', u) pairs in the first half, its
+(v, w) pairs in the second) and node-major FP32 (the four fields of a node together) at every
+order except 7 nodes per line, where it is laid out like the state.
 
 The emulation checks that every (element, node slot, field) gets its own location inside the
 values the context allocates (4 fs for the state and the LSRK register, 7 fs with Q), that every
-access stays inside its buffer resource's range, and that the lanes past the workgroup's elements
-touch only their own workgroup's range.
+access stays inside its buffer resource's range (the lanes past the workgroup's elements
+included, whose resources span only the workgroup's own elements' pieces).
 """
 import pytest
 
+P = 4096
 SHAPES = {2: (32, 16), 3: (8, 16), 4: (4, 8), 5: (2, 4), 6: (2, 1), 7: (2, 1), 8: (1, 2)}
+GROUP = {2: 512, 3: 128, 4: 64, 5: 32, 6: 18, 7: 10, 8: 8}
 
 
 def elem_stride(n):
@@ -26,7 +29,7 @@ def elem_stride(n):
 
 
 def field_slots(E, n):
-    return (E * elem_stride(n) + 2047) // 2048 * 2048 + 256
+    return -(-E // GROUP[n]) * P
 
 
 def du_layout(n, fp64):
@@ -36,47 +39,50 @@ def du_layout(n, fp64):
 
 
 def lane_accesses(n, epb, E, e0, tid, fs):
-    """The lane's element (or None past the workgroup's elements) and, per access, (absolute
-    value index, resource start, resource end) for: the state fields, the Q fields, and the LSRK
-    register in both precisions."""
-    es = elem_stride(n)
+    """The lane's element (or None past the workgroup's elements) and, per register kind, the
+    accesses (absolute value index, resource start, resource end) of: the state fields, the Q
+    fields, and the LSRK register in both precisions."""
+    es, G = elem_stride(n), GROUP[n]
     ne = min(E - e0, epb)
     le, node = divmod(tid, es)
     e = e0 + le
     io = le < epb and e < E
     ge = e if io else min(e0 + epb - 1, E - 1)
+    g, eg = divmod(e0, G)
+    gi = (ge - e0) * es + node
     out = {}
-    # state: resource over the run of the workgroup's blocks; gi + f * es
-    base, size = e0 * 4 * es, ne * 4 * es
-    gi = (ge - e0) * 4 * es + node
-    out["state"] = [(base + gi + f * es, base, base + size) for f in range(4)]
-    # Q fields: blocks of 3 es from 4 fs
-    qb, qs = 4 * fs + e0 * 3 * es, ne * 3 * es
-    qi = (ge - e0) * 3 * es + node
-    out["q"] = [(qb + qi + a * es, qb, qb + qs) for a in range(3)]
+    base = g * 4 * P + eg * es
+    out["state"] = [(base + gi + f * P, base, base + 3 * P + ne * es) for f in range(4)]
+    qb = 4 * fs + g * 3 * P + eg * es
+    out["q"] = [(qb + gi + a * P, qb, qb + 2 * P + ne * es) for a in range(3)]
     for fp64 in (True, False):
         kind = du_layout(n, fp64)
         vsize = 8 if fp64 else 4
-        lo, hi = e0 * 4 * es, e0 * 4 * es + ne * 4 * es           # in register values
+        d0 = g * 4 * P                                           # the group's block
         if kind == "field":
-            idx = [lo + gi + f * es for f in range(4)]
+            lo = d0 + eg * es
+            acc = [(lo + gi + f * P, lo, lo + 3 * P + ne * es) for f in range(4)]
         elif kind == "node":
-            off = ((ge - e0) * es + node) * 16                      # bytes from the base
-            assert off + 16 <= ne * 4 * es * vsize
-            idx = [lo + off // vsize + f for f in range(4)]
+            lo = d0 + eg * es * 4
+            off = gi * 16                                        # bytes from the base
+            acc = [(lo + off // vsize + f, lo, lo + ne * es * 16 // vsize) for f in range(4)]
         else:
-            off = ((ge - e0) * 2 * es + node) * 16
-            assert off + es * 16 + 16 <= ne * 4 * es * vsize
-            idx = [lo + off // vsize + f if f < 2 else lo + (off + es * 16) // vsize + f - 2 for f in range(4)]
-        out["du64" if fp64 else "du32"] = [(i, lo, hi) for i in idx]
+            lo = d0 + eg * es * 2
+            off = gi * 16
+            hi = lo + (2 * P * 8 + ne * es * 16) // vsize
+            acc = [(lo + off // vsize + f, lo, hi) if f < 2 else (lo + (off + P * 16) // vsize + f - 2, lo, hi)
+                   for f in range(4)]
+        out["du64" if fp64 else "du32"] = acc
     return (e if io else None), node, out
 
 
 @pytest.mark.parametrize("n", sorted(SHAPES))
 def test_registers_one_location_per_value(n):
     es = elem_stride(n)
+    assert GROUP[n] * es <= P
     for epb in SHAPES[n]:
-        for E in (1, epb, 3 * epb + 1, 7):
+        assert GROUP[n] % epb == 0
+        for E in (1, epb, 3 * epb + 1, 7, GROUP[n] + 1, 2 * GROUP[n] + epb):
             fs = field_slots(E, n)
             nt = (epb * es + 63) // 64 * 64
             seen = {k: {} for k in ("state", "q", "du64", "du32")}
@@ -96,18 +102,20 @@ def test_registers_one_location_per_value(n):
                 assert len(set(m.values())) == len(m), (kind, n, epb, E)
 
 
-def test_state_slots_match_the_element_blocked_rule():
+def test_state_slots_match_the_group_blocked_rule():
     """The state and Q slots the lanes address are dg3d_device.hpp's reg_slot:
-    (4 e + f) ES + node for f < 4, 4 fs + (3 e + f - 4) ES + node for the Q fields."""
-    n, epb, E = 5, 2, 5
-    es = elem_stride(n)
-    fs = field_slots(E, n)
-    for e0 in range(0, E, epb):
-        for tid in range((epb * es + 63) // 64 * 64):
-            e, node, acc = lane_accesses(n, epb, E, e0, tid, fs)
-            if e is None:
-                continue
-            for f, (i, _, _) in enumerate(acc["state"]):
-                assert i == (4 * e + f) * es + node
-            for a, (i, _, _) in enumerate(acc["q"]):
-                assert i == 4 * fs + (3 * e + a) * es + node
+    (e / G) 4P + f P + (e % G) ES + node for f < 4 and 4 fs + (e / G) 3P + (f - 4) P + (e % G) ES
+    + node for the Q fields, at every order with a partial last group."""
+    for n, (epb0, _) in SHAPES.items():
+        es, G = elem_stride(n), GROUP[n]
+        E = 2 * G + epb0
+        fs = field_slots(E, n)
+        for e0 in range(0, E, epb0):
+            for tid in range((epb0 * es + 63) // 64 * 64):
+                e, node, acc = lane_accesses(n, epb0, E, e0, tid, fs)
+                if e is None:
+                    continue
+                for f, (i, _, _) in enumerate(acc["state"]):
+                    assert i == (e // G) * 4 * P + f * P + (e % G) * es + node
+                for a, (i, _, _) in enumerate(acc["q"]):
+                    assert i == 4 * fs + (e // G) * 3 * P + a * P + (e % G) * es + node
