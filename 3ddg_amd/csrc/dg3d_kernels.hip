@@ -528,12 +528,13 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
 #pragma unroll
   for (int f = 0; f < 4; ++f) q[f] = bload<double>(rq, gi + f * P);
   // The LSRK register is never read across a face, so its layout within a group's block of 4 P
-  // values is free. A lane can move its node's share in 16-byte accesses: the FP64 register
+  // values is free. A lane moves its node's share in 16-byte accesses: the FP64 register
   // pair-major (the group's (rho', u) pairs in the block's first half, its (v, w) pairs in the
-  // second: two accesses), the FP32 register node-major (four fields in one access). Otherwise it
-  // is laid out like the state.
-  constexpr bool kDuNodeMajor = NN != 7 && std::is_same<DUT, float>::value;
-  constexpr bool kDuPairMajor = NN != 7 && std::is_same<DUT, double>::value;
+  // second: two accesses), the FP32 register node-major (four fields in one access). Laid out
+  // like the state instead, it ran 1.7-4 % slower in FP64 (64^3 N=4 and N=7, 48^3 N=6) and
+  // 0.8-2.3 % slower in FP32.
+  constexpr bool kDuNodeMajor = std::is_same<DUT, float>::value;
+  constexpr bool kDuPairMajor = std::is_same<DUT, double>::value;
   Rsrc rdu;
   uint32_t du_off = 0;
   if constexpr (EPI == EPI_LSRK) {
@@ -577,10 +578,11 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   // or the workgroup has one element, the element is the same on every lane of a wave that
   // holds any of its slots (lanes past the workgroup's elements return before the epilogue),
   // and its record is read once per wave through the scalar cache, which keeps these loads off
-  // the vector memory pipeline that the state and the neighbour gathers use. The one-element
-  // shape of 7 nodes per line keeps the vector loads: it ran 5 % slower with the scalar ones
+  // the vector memory pipeline that the state and the neighbour gathers use. The FP32 one-element
+  // shape of 7 nodes per line keeps the vector loads: it ran 1.5 % slower with the scalar ones
+  // (FP64: 4 % faster)
   double mat[6];
-  if constexpr (ES % 64 == 0 || (EPB == 1 && NN != 7)) {
+  if constexpr (ES % 64 == 0 || (EPB == 1 && (NN != 7 || std::is_same<CT, double>::value))) {
     typedef const __attribute__((address_space(4))) double* ConstPtr;  // read-only in the kernel
     const int lw = __builtin_amdgcn_readfirstlane(le);
     const ConstPtr pm = (ConstPtr)(m.mat + (size_t)((lw < EPB && e0 + lw < E) ? e0 + lw : 0) * kMatStride);

@@ -9,8 +9,8 @@ workgroup reaches them through buffer resources that start at its first element'
 
 The LSRK register is never read across a face, so its layout inside a group's block of 4P values
 is the kernel's choice: pair-major FP64 (the group's (rho', u) pairs in the first half, its
-(v, w) pairs in the second) and node-major FP32 (the four fields of a node together) at every
-order except 7 nodes per line, where it is laid out like the state.
+(v, w) pairs in the second) and node-major FP32 (the four fields of a node together). The emulation
+also covers the register laid out like the state ("field"), the kernel's earlier form.
 
 The emulation checks that every (element, node slot, field) gets its own location inside the
 values the context allocates (4 fs for the state and the LSRK register, 7 fs with Q), that every
@@ -33,9 +33,7 @@ def field_slots(E, n):
 
 
 def du_layout(n, fp64):
-    if fp64:
-        return "pair" if n != 7 else "field"
-    return "node" if n != 7 else "field"
+    return "pair" if fp64 else "node"
 
 
 def lane_accesses(n, epb, E, e0, tid, fs):
@@ -55,8 +53,7 @@ def lane_accesses(n, epb, E, e0, tid, fs):
     out["state"] = [(base + gi + f * P, base, base + 3 * P + ne * es) for f in range(4)]
     qb = 4 * fs + g * 3 * P + eg * es
     out["q"] = [(qb + gi + a * P, qb, qb + 2 * P + ne * es) for a in range(3)]
-    for fp64 in (True, False):
-        kind = du_layout(n, fp64)
+    for fp64, kind in ((True, du_layout(n, True)), (False, du_layout(n, False)), (True, "field")):
         vsize = 8 if fp64 else 4
         d0 = g * 4 * P                                           # the group's block
         if kind == "field":
@@ -72,7 +69,7 @@ def lane_accesses(n, epb, E, e0, tid, fs):
             hi = lo + (2 * P * 8 + ne * es * 16) // vsize
             acc = [(lo + off // vsize + f, lo, hi) if f < 2 else (lo + (off + P * 16) // vsize + f - 2, lo, hi)
                    for f in range(4)]
-        out["du64" if fp64 else "du32"] = acc
+        out[("du64" if fp64 else "du32") + ("f" if kind == "field" else "")] = acc
     return (e if io else None), node, out
 
 
@@ -85,7 +82,7 @@ def test_registers_one_location_per_value(n):
         for E in (1, epb, 3 * epb + 1, 7, GROUP[n] + 1, 2 * GROUP[n] + epb):
             fs = field_slots(E, n)
             nt = (epb * es + 63) // 64 * 64
-            seen = {k: {} for k in ("state", "q", "du64", "du32")}
+            seen = {k: {} for k in ("state", "q", "du64", "du32", "du64f")}
             for e0 in range(0, E, epb):
                 for tid in range(nt):
                     e, node, acc = lane_accesses(n, epb, E, e0, tid, fs)
