@@ -309,6 +309,66 @@ struct TileLine {
   }
 };
 
+// Shape of the matrix-core contraction of a workgroup (see contract_mfma): block stride BS, P
+// blocks per 16x16 operand, C columns (lines), TC columns per tile, T tiles, W waves, and at most
+// TMAX tiles per wave (waves take tiles round-robin).
+template <int NN, int EPB, int NT>
+struct MfmaShape {
+  static constexpr int BS = NN <= 4 ? 4 : 8;
+  static constexpr int P = 16 / BS;
+  static constexpr int C = EPB * 6 * NN * NN;
+  static constexpr int TC = 16 * P;
+  static constexpr int T = (C + TC - 1) / TC;
+  static constexpr int W = NT / 64;
+  static constexpr int TMAX = (T + W - 1) / W;
+};
+
+// Orders whose split-BF16 tiles read their column descriptors from a table (n not a power of two,
+// where finding a column's line costs divisions by n and n^2): with the table, 64^3 N=4 ran 4.3 %
+// faster and 48^3 N=5 the same; at n = 8 (divisions by powers of two) the table cost 1.3 %.
+template <int NN>
+constexpr bool kColumnDescriptors = (NN & (NN - 1)) != 0;
+
+// Column descriptors of the split-BF16 contraction, one 32-bit word per column of the padded
+// tiles (column_line's mapping, found at compile time instead of per tile): bits 0-11 the line's
+// first node in sF, bits 12-24 in sOut, bits 25-26 the axis (node stride NN^2, NN or 1), bit 27 a
+// P/rho0 line (entered shifted by its middle value), bit 31 a column that exists.
+template <int NN, int EPB>
+struct ColumnTable {
+  static constexpr int NP = NN * NN * NN, NF2 = NN * NN;
+  static constexpr int BS = NN <= 4 ? 4 : 8, TC = 16 * (16 / BS), C = EPB * 6 * NF2;
+  static constexpr int N = (C + TC - 1) / TC * TC;
+  static_assert(EPB * 4 * NP <= 4096 && EPB * 6 * NP <= 8192, "descriptor fields");
+  uint32_t v[N];
+  constexpr ColumnTable() : v() {
+    for (int col = 0; col < N; ++col) {
+      if (col >= C) continue;  // 0: no column
+      const int le = col / (6 * NF2), rr = col % (6 * NF2), af = rr / NF2, fn = rr % NF2;
+      const int a = af >> 1, f = af & 1, b = fn / NN, c = fn % NN;
+      const int base = (a == 0) ? b * NN + c : (a == 1 ? b * NF2 + c : b * NF2 + c * NN);
+      const int fo = le * 4 * NP + (f == 0 ? 0 : 1 + a) * NP + base;
+      const int oo = (le * 6 + af) * NP + base;
+      v[col] = (uint32_t)fo | ((uint32_t)oo << 12) | ((uint32_t)a << 25) | ((uint32_t)(f == 0) << 27) | (1u << 31);
+    }
+  }
+};
+template <int NN, int EPB>
+__device__ inline constexpr ColumnTable<NN, EPB> kColumnTable{};
+
+// The descriptors of the columns a lane maps in its wave's tiles (0 past the last tile), read with
+// the stage's other global loads
+template <int NN, int EPB, int NT>
+__device__ __forceinline__ void column_descriptors(int tid, uint32_t* d) {
+  using M = MfmaShape<NN, EPB, NT>;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int bd = (4 * (lane >> 4)) / M::BS;
+#pragma unroll
+  for (int k = 0; k < M::TMAX; ++k) {
+    const int t = wave + k * M::W;
+    d[k] = t < M::T ? kColumnTable<NN, EPB>.v[t * M::TC + 16 * bd + (lane & 15)] : 0u;
+  }
+}
+
 // The workgroup's derivative contractions as one GEMM on the matrix cores:
 // out[i][col] = sum_m A[i][m] line_col[m], one column per (element, axis, field, face node);
 // P/rho0 lines enter shifted by their middle value, as in the VALU contraction of the FP32 stage.
@@ -324,7 +384,8 @@ struct TileLine {
 //   16x16x4f32:        A[j][k = g], B[k = g][j], D[4g + r][j]   (K step s covers 4s..4s+3)
 //   16x16x16bf16_1k:   A[j][k = 4g + t], B[k = 4g + t][j], D[4g + r][j]
 template <int NN, int EPB, int NT, int CONTRACT>
-__device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf, float* sOut, int tid) {
+__device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf, float* sOut, int tid,
+                                              const uint32_t* cdesc) {
   constexpr int BS = NN <= 4 ? 4 : 8;
   constexpr int P = 16 / BS;
   constexpr int C = EPB * 6 * NN * NN;
@@ -378,11 +439,55 @@ __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf,
       am[q] = m;
       al[q] = l;
     }
-    for (int t = wave; t < T; t += W) {
-      const TileLine<NN> ln(t * TC + 16 * bd + j, C, sF);
+    if constexpr (!kColumnDescriptors<NN>) {
+      // n a power of two: the column's line found per tile (divisions by powers of two)
+      for (int t = wave; t < T; t += W) {
+        const TileLine<NN> ln(t * TC + 16 * bd + j, C, sF);
+        bf16x2v h0, m0, l0, h1, m1, l1;
+        split3x2(float2v{ln.load(sF, md), ln.load(sF, md + 1)}, h0, m0, l0);
+        split3x2(float2v{ln.load(sF, md + 2), ln.load(sF, md + 3)}, h1, m1, l1);
+        const bf16x4 bh = join4(h0, h1), bm = join4(m0, m1), bl = join4(l0, l1);
+        const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+        floatx4 acc = zero;
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bm, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bm, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bh, acc, 0, 0, 0);
+        // the dominant hi x hi group from a zero accumulator, then one round-to-nearest add: fed
+        // the small terms as its accumulator input, the matrix core biased the sum (by about
+        // half its mean rounding error, always the same way), which grew linearly over a run
+        const floatx4 hh = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, zero, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ln.store(sOut, md + r, hh[r] + acc[r]);
+      }
+      return;
+    }
+    // the lane's rows md..md+3 of its line: which exist does not change from tile to tile
+    bool vr[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) vr[r] = md + r < NN;
+#pragma unroll
+    for (int k = 0; k < MfmaShape<NN, EPB, NT>::TMAX; ++k) {
+      if (wave + k * W >= T) break;  // wave-uniform
+      const uint32_t d = cdesc[k];
+      const bool ok = (d >> 31) != 0u;
+      const int fo = (int)(d & 0xFFFu), oo = (int)((d >> 12) & 0x1FFFu);
+      const uint32_t ax = (d >> 25) & 3u;
+      const int ls = ax == 0u ? NN * NN : (ax == 1u ? NN : 1);
+      // a P/rho0 line enters shifted by its middle value, as in the VALU contraction
+      const float sh = ((d >> 27) & 1u) ? sF[fo + (NN / 2) * ls] : 0.f;
+      bool in[4];
+      float x[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        in[r] = ok && vr[r];
+        const float y = sF[in[r] ? fo + (md + r) * ls : 0];
+        x[r] = in[r] ? y - sh : 0.f;
+      }
       bf16x2v h0, m0, l0, h1, m1, l1;
-      split3x2(float2v{ln.load(sF, md), ln.load(sF, md + 1)}, h0, m0, l0);
-      split3x2(float2v{ln.load(sF, md + 2), ln.load(sF, md + 3)}, h1, m1, l1);
+      split3x2(float2v{x[0], x[1]}, h0, m0, l0);
+      split3x2(float2v{x[2], x[3]}, h1, m1, l1);
       const bf16x4 bh = join4(h0, h1), bm = join4(m0, m1), bl = join4(l0, l1);
       const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
       floatx4 acc = zero;
@@ -396,7 +501,8 @@ __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf,
       // its mean rounding error, always the same way), which grew linearly over a run
       const floatx4 hh = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, zero, 0, 0, 0);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) ln.store(sOut, md + r, hh[r] + acc[r]);
+      for (int r = 0; r < 4; ++r)
+        if (in[r]) sOut[oo + (md + r) * ls] = hh[r] + acc[r];
     }
   }
 }
@@ -597,6 +703,10 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   const CT c = (CT)mat[MAT_C], r0 = (CT)mat[MAT_RHO0];
   const CT ba = (CT)mat[MAT_BA], sg = (CT)mat[MAT_SIGMA];
   const CT ir0 = (CT)mat[MAT_IRHO0];
+  // the split-BF16 contraction's column descriptors (a small table every workgroup reads)
+  constexpr bool kDesc = CONTRACT == 2 && kColumnDescriptors<NN>;
+  uint32_t cdesc[kDesc ? MfmaShape<NN, EPB, NT>::TMAX : 1] = {};
+  if constexpr (kDesc) column_descriptors<NN, EPB, NT>(tid, cdesc);
   double qq[3] = {0.0, 0.0, 0.0};
   CT dcoef = CT(0);
   if constexpr (LDG) {
@@ -808,7 +918,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   }
   // the matrix-core contraction reads only phase 1's LDS (sF, sA) and writes its own array, so
   // it runs beside the flux pairs, before the barrier that phase 3 needs anyway
-  if constexpr (CONTRACT != 0) contract_mfma<NN, EPB, NT, CONTRACT>(sF, sA, sOut, tid);
+  if constexpr (CONTRACT != 0) contract_mfma<NN, EPB, NT, CONTRACT>(sF, sA, sOut, tid, cdesc);
   __syncthreads();
 
   if (!io) return;

@@ -130,3 +130,53 @@ def test_tile_map_covers_every_line_once_in_bounds(n):
             got, writes = emulate(n, epb, mode, A, sF)
             assert (writes == 1).all(), (n, epb, mode, writes.min(), writes.max())
             np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def descriptor(n, epb, col):
+    """The split-BF16 contraction's column descriptor (ColumnTable in dg3d_kernels.hip): bits 0-11
+    the line's first node in sF, 12-24 in sOut, 25-26 the axis, 27 a P/rho0 line, 31 a column
+    that exists; 0 for the padding columns of the last tile."""
+    NF2 = n * n
+    if col >= epb * 6 * NF2:
+        return 0
+    fo, oo, _ = column_line(n, col)
+    a = (col % (6 * NF2)) // NF2 >> 1
+    pressure = ((col % (6 * NF2)) // NF2) % 2 == 0
+    assert fo < 4096 and oo < 8192
+    return fo | oo << 12 | a << 25 | int(pressure) << 27 | 1 << 31
+
+
+@pytest.mark.parametrize("n", sorted(SHAPES))
+def test_column_descriptors_decode_to_the_column_lines(n):
+    """What the split-BF16 tile loop decodes from a lane's descriptors equals column_line's line
+    (first nodes, stride, shift flag) for every column of every tile a wave takes, and the padding
+    columns of the last tile decode as absent."""
+    NF2 = n * n
+    for epb in SHAPES[n]:
+        NT = (epb * elem_stride(n) + 63) // 64 * 64
+        W = NT // 64
+        BS = 4 if n <= 4 else 8
+        TC = 16 * (16 // BS)
+        C = epb * 6 * NF2
+        T = (C + TC - 1) // TC
+        TMAX = (T + W - 1) // W
+        seen = set()
+        for wave in range(W):
+            for lane in range(64):
+                bd = (4 * (lane >> 4)) // BS
+                for k in range(TMAX):
+                    t = wave + k * W
+                    if t >= T:
+                        break
+                    col = t * TC + 16 * bd + (lane & 15)
+                    d = descriptor(n, epb, col)
+                    ok = d >> 31 & 1
+                    assert ok == (col < C)
+                    if not ok:
+                        continue
+                    fo, oo, ax, pr = d & 0xFFF, d >> 12 & 0x1FFF, d >> 25 & 3, d >> 27 & 1
+                    ls = NF2 if ax == 0 else (n if ax == 1 else 1)
+                    assert (fo, oo, ls) == column_line(n, col)
+                    assert pr == int(((col % (6 * NF2)) // NF2) % 2 == 0)
+                    seen.add(col)
+        assert seen == set(range(C))
