@@ -1060,12 +1060,32 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT), (waves_target<NN, CT, MINW
 using ConstLaunchPtr = const __attribute__((address_space(4))) RhsLaunch*;
 using ConstTaskPtr = const __attribute__((address_space(4))) BatchTask*;
 
+// The tasks of a launch in per-XCD slabs: the hardware sends workgroup w to XCD w mod 8, and XCD x
+// takes the x-th eighth of the tasks, in order, so that a solve's neighbouring element groups meet
+// in one XCD's L2 (in the hardware order they spread over all eight). One task per workgroup:
+// workgroup w's task. Persistent (grid >= 8): the workgroups of XCD x take its slab's tasks
+// round-robin. With the slabs, the 256-solve batch stage went from 1.084 to 1.001 ms mixed and from
+// 1.34 to 1.27-1.29 ms FP64 (alternating runs on one box).
+__device__ __forceinline__ void xcd_slab(uint32_t ntasks, uint32_t x, uint32_t* first, uint32_t* end) {
+  const uint32_t per = ntasks >> 3, rem = ntasks & 7u;
+  *first = x * per + (x < rem ? x : rem);
+  *end = *first + per + (x < rem ? 1u : 0u);
+}
+
 template <int NN, int EPB, typename CT, typename DUT, bool PERSIST, bool LATE>
 __global__ void __launch_bounds__((Geo<NN, EPB>::NT)) k_rhs_batch(const BatchLaunch B) {
   __shared__ StageSmem<NN, EPB, CT, 0> sm;
   const ConstTaskPtr tasks = (ConstTaskPtr)B.tasks;
+  const uint32_t w = blockIdx.x, x = w & 7u;
   if constexpr (PERSIST) {
-    for (int t = blockIdx.x; t < B.ntasks; t += gridDim.x) {
+    // fewer than 8 workgroups leave some XCD without one: then the plain order
+    uint32_t t0 = w, t1 = (uint32_t)B.ntasks, step = gridDim.x;
+    if (gridDim.x >= 8) {
+      xcd_slab((uint32_t)B.ntasks, x, &t0, &t1);
+      step = (gridDim.x - x + 7) >> 3;  // the workgroups on XCD x
+      t0 += w >> 3;
+    }
+    for (uint32_t t = t0; t < t1; t += step) {
       __syncthreads();  // the previous task's phase 3 still reads LDS
       // the lane index is re-read inside the loop: without it the compiler hoists the
       // task-independent lane arithmetic out of the loop and holds it in registers across tasks
@@ -1076,8 +1096,11 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT)) k_rhs_batch(const BatchLau
       stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI_LSRK, DUT, 0, LATE>(L, L.mesh, L.ext.trace, e0, tid, sm);
     }
   } else {
-    const int t = blockIdx.x;
-    if (t >= B.ntasks) return;
+    // the grid is one workgroup per task
+    uint32_t first, end;
+    xcd_slab(gridDim.x, x, &first, &end);
+    const uint32_t t = first + (w >> 3);
+    if (t >= (uint32_t)B.ntasks) return;
     const int solve = tasks[t].solve, e0 = tasks[t].e0;
     const RhsLaunch& L = *(const RhsLaunch*)((ConstLaunchPtr)(B.records + (int64_t)solve * kBatchRecords + B.record));
     stage_body<NN, EPB, Geo<NN, EPB>::NT, CT, EPI_LSRK, DUT, 0, LATE>(L, L.mesh, L.ext.trace, e0, (int)threadIdx.x,
