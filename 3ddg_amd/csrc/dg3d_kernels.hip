@@ -309,40 +309,39 @@ struct TileLine {
   }
 };
 
-// Shape of the matrix-core contraction of a workgroup (see contract_mfma): block stride BS, P
-// blocks per 16x16 operand, C columns (lines), TC columns per tile, T tiles, W waves, and at most
-// TMAX tiles per wave (waves take tiles round-robin).
+// Orders whose split-BF16 contraction takes one line per lane (contract_lines_bf16); at 8 nodes per
+// line the 16x16x16 tiles of contract_mfma, whose two 8-row blocks fill each tile, ran 1.7 %
+// faster (64^3 N=7), while at 5 and 6 nodes per line the lines ran 2.8 and 2.4 % faster than
+// those tiles (64^3 N=4, 48^3 N=5).
+template <int NN>
+constexpr bool kLinesPerLane = NN < 8;
+
+// Shape of the split-BF16 contraction of a workgroup (contract_lines_bf16): C lines, in groups of
+// 64 (one line per lane of a wave), G groups taken round-robin by the W waves, at most GMAX per
+// wave; RB row blocks and KB K blocks of 4 cover a line of NN nodes.
 template <int NN, int EPB, int NT>
-struct MfmaShape {
-  static constexpr int BS = NN <= 4 ? 4 : 8;
-  static constexpr int P = 16 / BS;
+struct LineShape {
   static constexpr int C = EPB * 6 * NN * NN;
-  static constexpr int TC = 16 * P;
-  static constexpr int T = (C + TC - 1) / TC;
+  static constexpr int G = (C + 63) / 64;
   static constexpr int W = NT / 64;
-  static constexpr int TMAX = (T + W - 1) / W;
+  static constexpr int GMAX = (G + W - 1) / W;
+  static constexpr int RB = NN > 4 ? 2 : 1;
+  static constexpr int KB = RB;
 };
 
-// Orders whose split-BF16 tiles read their column descriptors from a table (n not a power of two,
-// where finding a column's line costs divisions by n and n^2): with the table, 64^3 N=4 ran 4.3 %
-// faster and 48^3 N=5 the same; at n = 8 (divisions by powers of two) the table cost 1.3 %.
-template <int NN>
-constexpr bool kColumnDescriptors = (NN & (NN - 1)) != 0;
-
-// Column descriptors of the split-BF16 contraction, one 32-bit word per column of the padded
-// tiles (column_line's mapping, found at compile time instead of per tile): bits 0-11 the line's
-// first node in sF, bits 12-24 in sOut, bits 25-26 the axis (node stride NN^2, NN or 1), bit 27 a
-// P/rho0 line (entered shifted by its middle value), bit 31 a column that exists.
+// Line descriptors of the split-BF16 contraction, one 32-bit word per line of the workgroup, padded
+// to whole groups of 64 (a line is one (element, axis, field, face node) with the field P/rho0 or
+// m_axis; the mapping of column_line, found at compile time): bits 0-11 the line's first node in
+// sF, bits 12-24 in sOut, bits 25-26 the axis (node stride NN^2, NN or 1), bit 27 a P/rho0 line
+// (entered shifted by its middle value), bit 31 a line that exists (0: padding).
 template <int NN, int EPB>
-struct ColumnTable {
-  static constexpr int NP = NN * NN * NN, NF2 = NN * NN;
-  static constexpr int BS = NN <= 4 ? 4 : 8, TC = 16 * (16 / BS), C = EPB * 6 * NF2;
-  static constexpr int N = (C + TC - 1) / TC * TC;
+struct LineTable {
+  static constexpr int NP = NN * NN * NN, NF2 = NN * NN, C = EPB * 6 * NF2;
+  static constexpr int N = (C + 63) / 64 * 64;
   static_assert(EPB * 4 * NP <= 4096 && EPB * 6 * NP <= 8192, "descriptor fields");
   uint32_t v[N];
-  constexpr ColumnTable() : v() {
-    for (int col = 0; col < N; ++col) {
-      if (col >= C) continue;  // 0: no column
+  constexpr LineTable() : v() {
+    for (int col = 0; col < C; ++col) {
       const int le = col / (6 * NF2), rr = col % (6 * NF2), af = rr / NF2, fn = rr % NF2;
       const int a = af >> 1, f = af & 1, b = fn / NN, c = fn % NN;
       const int base = (a == 0) ? b * NN + c : (a == 1 ? b * NF2 + c : b * NF2 + c * NN);
@@ -353,19 +352,89 @@ struct ColumnTable {
   }
 };
 template <int NN, int EPB>
-__device__ inline constexpr ColumnTable<NN, EPB> kColumnTable{};
+__device__ inline constexpr LineTable<NN, EPB> kLineTable{};
 
-// The descriptors of the columns a lane maps in its wave's tiles (0 past the last tile), read with
+// The descriptors of the lines a lane takes in its wave's groups (0 past the last group), read with
 // the stage's other global loads
 template <int NN, int EPB, int NT>
-__device__ __forceinline__ void column_descriptors(int tid, uint32_t* d) {
-  using M = MfmaShape<NN, EPB, NT>;
+__device__ __forceinline__ void line_descriptors(int tid, uint32_t* d) {
+  using M = LineShape<NN, EPB, NT>;
   const int lane = tid & 63, wave = tid >> 6;
-  const int bd = (4 * (lane >> 4)) / M::BS;
 #pragma unroll
-  for (int k = 0; k < M::TMAX; ++k) {
-    const int t = wave + k * M::W;
-    d[k] = t < M::T ? kColumnTable<NN, EPB>.v[t * M::TC + 16 * bd + (lane & 15)] : 0u;
+  for (int k = 0; k < M::GMAX; ++k) {
+    const int g = wave + k * M::W;
+    d[k] = g < M::G ? kLineTable<NN, EPB>.v[g * 64 + lane] : 0u;
+  }
+}
+
+__device__ __forceinline__ floatx4 mfma4(bf16x4 a, bf16x4 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// The split-BF16 derivative contraction on the matrix cores, one line per lane: out = A line for
+// every line of the workgroup (P/rho0 lines entered shifted by their middle value, as in the VALU
+// contraction of the FP32 stage). v_mfma_f32_4x4x4bf16_1k computes 16 independent 4x4x4 products;
+// its lane layout, checked on the device before use: lane 4b + i holds row i of block b's A operand
+// (its 4 K values), lane 4b + j holds column j of block b's B operand (4 K values), and register r
+// of lane 4b + j receives row r, column j of block b's result. Every block takes the same 4x4 piece
+// of A, so each lane's B column is a K block of its own line and its results are rows of its own
+// line: one piece for n <= 4, four for n > 4 (row blocks R and K blocks of 4, zero past n). A
+// (in sAp, split once per workgroup) and the lines are split into three bf16 pieces each,
+// hi + mid + lo, exactly; per row block the five products of combined order 1 and 2 are accumulated
+// in FP32, smallest first, through the matrix core's accumulator, and the hi x hi products of each
+// K block come from a zero accumulator and are added with FP32 adds (fed the small terms as its
+// accumulator input, the matrix core biased the sum).
+template <int NN, int EPB, int NT>
+__device__ __forceinline__ void contract_lines_bf16(const float* sF, const bf16x4* sAp, float* sOut, int tid,
+                                                    const uint32_t* ldesc) {
+  using M = LineShape<NN, EPB, NT>;
+  constexpr int RB = M::RB, KB = M::KB;
+  const int wave = tid >> 6, ia = tid & 3;
+#pragma unroll
+  for (int k = 0; k < M::GMAX; ++k) {
+    if (wave + k * M::W >= M::G) break;  // wave-uniform
+    const uint32_t d = ldesc[k];
+    const bool ok = (d >> 31) != 0u;
+    const int fo = ok ? (int)(d & 0xFFFu) : 0, oo = (int)((d >> 12) & 0x1FFFu);
+    const uint32_t ax = (d >> 25) & 3u;
+    const int ls = ax == 0u ? NN * NN : (ax == 1u ? NN : 1);
+    float x[4 * KB];
+#pragma unroll
+    for (int m = 0; m < 4 * KB; ++m) x[m] = m < NN ? sF[fo + m * ls] : 0.f;
+    const float sh = ((d >> 27) & 1u) ? x[NN / 2] : 0.f;
+#pragma unroll
+    for (int m = 0; m < NN; ++m) x[m] = ok ? x[m] - sh : 0.f;
+    bf16x4 bh[KB], bm[KB], bl[KB];
+#pragma unroll
+    for (int K = 0; K < KB; ++K) {
+      bf16x2v h0, m0, l0, h1, m1, l1;
+      split3x2(float2v{x[4 * K], x[4 * K + 1]}, h0, m0, l0);
+      split3x2(float2v{x[4 * K + 2], x[4 * K + 3]}, h1, m1, l1);
+      bh[K] = join4(h0, h1);
+      bm[K] = join4(m0, m1);
+      bl[K] = join4(l0, l1);
+    }
+#pragma unroll
+    for (int R = 0; R < RB; ++R) {
+      const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+      floatx4 acc = zero, hh = zero;
+#pragma unroll
+      for (int K = 0; K < KB; ++K) {
+        const bf16x4* ap = sAp + (R * KB + K) * 12 + ia;  // pieces hi, mid, lo 4 apart
+        const bf16x4 ah = ap[0], am = ap[4], al = ap[8];
+        acc = mfma4(ah, bl[K], acc);
+        acc = mfma4(am, bm[K], acc);
+        acc = mfma4(al, bh[K], acc);
+        acc = mfma4(ah, bm[K], acc);
+        acc = mfma4(am, bh[K], acc);
+        const floatx4 h = mfma4(ah, bh[K], zero);
+        if (K == 0) hh = h;
+        else hh = hh + h;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (ok && 4 * R + r < NN) sOut[oo + (4 * R + r) * ls] = hh[r] + acc[r];
+    }
   }
 }
 
@@ -377,15 +446,15 @@ __device__ __forceinline__ void column_descriptors(int tid, uint32_t* d) {
 // rows a lane holds (4g..4g+3, in both the K operand and the result) lie in one block. A tile
 // covers 16 columns per block (columns col0 + 16 b), and waves take tiles round-robin.
 //   CONTRACT 1: v_mfma_f32_16x16x4f32, FP32 operands, 4 K steps.
-//   CONTRACT 2: v_mfma_f32_16x16x16bf16_1k with A and the lines split into three bf16 pieces
-//               each; the five products of combined order 1 and 2 are accumulated in FP32,
-//               smallest first, and the hi x hi products from zero, then added to them.
+//   CONTRACT 2 (8 nodes per line; other orders take contract_lines_bf16): v_mfma_f32_16x16x16bf16_1k
+//               with A and the lines split into three bf16 pieces each; the five products of
+//               combined order 1 and 2 are accumulated in FP32, smallest first, and the hi x hi
+//               products from zero, then added to them.
 // Lane layouts (16x16 tile, lane l, g = l/16, j = l%16):
 //   16x16x4f32:        A[j][k = g], B[k = g][j], D[4g + r][j]   (K step s covers 4s..4s+3)
 //   16x16x16bf16_1k:   A[j][k = 4g + t], B[k = 4g + t][j], D[4g + r][j]
 template <int NN, int EPB, int NT, int CONTRACT>
-__device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf, float* sOut, int tid,
-                                              const uint32_t* cdesc) {
+__device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf, float* sOut, int tid) {
   constexpr int BS = NN <= 4 ? 4 : 8;
   constexpr int P = 16 / BS;
   constexpr int C = EPB * 6 * NN * NN;
@@ -429,6 +498,10 @@ __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf,
       for (int r = 0; r < 4; ++r) out.store(sOut, md + r, acc[r]);
     }
   } else {
+    // split-BF16 on v_mfma_f32_16x16x16bf16_1k (used at 8 nodes per line, where the two 8-row
+    // blocks fill the tile): A and the lines are split into three bf16 pieces each; the five
+    // products of combined order 1 and 2 are accumulated in FP32, smallest first, and the hi x hi
+    // products from a zero accumulator, then added to them
     bf16x4 ah, am, al;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -439,55 +512,11 @@ __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf,
       am[q] = m;
       al[q] = l;
     }
-    if constexpr (!kColumnDescriptors<NN>) {
-      // n a power of two: the column's line found per tile (divisions by powers of two)
-      for (int t = wave; t < T; t += W) {
-        const TileLine<NN> ln(t * TC + 16 * bd + j, C, sF);
-        bf16x2v h0, m0, l0, h1, m1, l1;
-        split3x2(float2v{ln.load(sF, md), ln.load(sF, md + 1)}, h0, m0, l0);
-        split3x2(float2v{ln.load(sF, md + 2), ln.load(sF, md + 3)}, h1, m1, l1);
-        const bf16x4 bh = join4(h0, h1), bm = join4(m0, m1), bl = join4(l0, l1);
-        const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
-        floatx4 acc = zero;
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bm, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bm, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bh, acc, 0, 0, 0);
-        // the dominant hi x hi group from a zero accumulator, then one round-to-nearest add: fed
-        // the small terms as its accumulator input, the matrix core biased the sum (by about
-        // half its mean rounding error, always the same way), which grew linearly over a run
-        const floatx4 hh = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, zero, 0, 0, 0);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) ln.store(sOut, md + r, hh[r] + acc[r]);
-      }
-      return;
-    }
-    // the lane's rows md..md+3 of its line: which exist does not change from tile to tile
-    bool vr[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) vr[r] = md + r < NN;
-#pragma unroll
-    for (int k = 0; k < MfmaShape<NN, EPB, NT>::TMAX; ++k) {
-      if (wave + k * W >= T) break;  // wave-uniform
-      const uint32_t d = cdesc[k];
-      const bool ok = (d >> 31) != 0u;
-      const int fo = (int)(d & 0xFFFu), oo = (int)((d >> 12) & 0x1FFFu);
-      const uint32_t ax = (d >> 25) & 3u;
-      const int ls = ax == 0u ? NN * NN : (ax == 1u ? NN : 1);
-      // a P/rho0 line enters shifted by its middle value, as in the VALU contraction
-      const float sh = ((d >> 27) & 1u) ? sF[fo + (NN / 2) * ls] : 0.f;
-      bool in[4];
-      float x[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        in[r] = ok && vr[r];
-        const float y = sF[in[r] ? fo + (md + r) * ls : 0];
-        x[r] = in[r] ? y - sh : 0.f;
-      }
+    for (int t = wave; t < T; t += W) {
+      const TileLine<NN> ln(t * TC + 16 * bd + j, C, sF);
       bf16x2v h0, m0, l0, h1, m1, l1;
-      split3x2(float2v{x[0], x[1]}, h0, m0, l0);
-      split3x2(float2v{x[2], x[3]}, h1, m1, l1);
+      split3x2(float2v{ln.load(sF, md), ln.load(sF, md + 1)}, h0, m0, l0);
+      split3x2(float2v{ln.load(sF, md + 2), ln.load(sF, md + 3)}, h1, m1, l1);
       const bf16x4 bh = join4(h0, h1), bm = join4(m0, m1), bl = join4(l0, l1);
       const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
       floatx4 acc = zero;
@@ -501,8 +530,7 @@ __device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf,
       // its mean rounding error, always the same way), which grew linearly over a run
       const floatx4 hh = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, zero, 0, 0, 0);
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        if (in[r]) sOut[oo + (md + r) * ls] = hh[r] + acc[r];
+      for (int r = 0; r < 4; ++r) ln.store(sOut, md + r, hh[r] + acc[r]);
     }
   }
 }
@@ -534,6 +562,8 @@ struct StageSmem {
   CT sK[EPB * 6 * kNK];                   // flux coefficients per face
   CT sIw[EPB * 3];                        // 1/width per element and axis
   float sOut[CONTRACT ? EPB * 6 * NP : 1];  // matrix-core contraction results
+  // split-BF16 contraction: the pieces of A, [row block][K block][hi, mid, lo][row in block]
+  bf16x4 sAp[CONTRACT == 2 && NN < 8 ? (NN > 4 ? 4 : 1) * 12 : 1];
   CT sQ[LDG ? EPB * 3 * NP : 1];          // LDG: Q_x, Q_y, Q_z per node
   CT sFq[LDG ? EPB * 6 * NF2 : 1];        // LDG: central trace of Q_axis per face node
   CT sA[NN * NN];
@@ -703,10 +733,10 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   const CT c = (CT)mat[MAT_C], r0 = (CT)mat[MAT_RHO0];
   const CT ba = (CT)mat[MAT_BA], sg = (CT)mat[MAT_SIGMA];
   const CT ir0 = (CT)mat[MAT_IRHO0];
-  // the split-BF16 contraction's column descriptors (a small table every workgroup reads)
-  constexpr bool kDesc = CONTRACT == 2 && kColumnDescriptors<NN>;
-  uint32_t cdesc[kDesc ? MfmaShape<NN, EPB, NT>::TMAX : 1] = {};
-  if constexpr (kDesc) column_descriptors<NN, EPB, NT>(tid, cdesc);
+  // the split-BF16 contraction's line descriptors (a small table every workgroup reads)
+  constexpr bool kLines = CONTRACT == 2 && kLinesPerLane<NN>;
+  uint32_t ldesc[kLines ? LineShape<NN, EPB, NT>::GMAX : 1] = {};
+  if constexpr (kLines) line_descriptors<NN, EPB, NT>(tid, ldesc);
   double qq[3] = {0.0, 0.0, 0.0};
   CT dcoef = CT(0);
   if constexpr (LDG) {
@@ -747,6 +777,13 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     if ((hdir & 1) == 0) hiw = sel3(axis, m.inv_w[0], m.inv_w[1], m.inv_w[2])[sel3(axis, ec[0], ec[1], ec[2])];
   }
   CT av = CT(0), l0v = CT(0), lnv = CT(0);
+  // split-BF16 contraction: entry (row, col) of A on the grid of its 4x4 pieces, zero past n
+  constexpr int kPW = 4 * LineShape<NN, EPB, NT>::KB;  // grid width
+  float apv = 0.f;
+  if constexpr (kLines) {
+    const int row = ot / kPW, col = ot - (ot / kPW) * kPW;
+    if (ot >= 0 && row < NN && col < NN) apv = L.opf[0][row * NN + col];
+  }
   if (ot >= 0 && ot < NN * NN) {
     if constexpr (std::is_same<CT, double>::value) av = L.opd[0][ot];
     else av = L.opf[0][ot];
@@ -858,6 +895,17 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     sL0[ot] = l0v;
     sLN[ot] = lnv;
   }
+  if constexpr (kLines) {
+    if (ot >= 0 && ot < kPW * kPW) {
+      const int row = ot / kPW, col = ot - (ot / kPW) * kPW;
+      short pc[3];
+      split3(apv, pc[0], pc[1], pc[2]);
+      short* sp = reinterpret_cast<short*>(sm.sAp);
+#pragma unroll
+      for (int pi = 0; pi < 3; ++pi)
+        sp[((((row >> 2) * (kPW / 4) + (col >> 2)) * 3 + pi) * 4 + (row & 3)) * 4 + (col & 3)] = pc[pi];
+    }
+  }
   __syncthreads();
 
   if constexpr (LATE) load_pre();
@@ -918,7 +966,8 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   }
   // the matrix-core contraction reads only phase 1's LDS (sF, sA) and writes its own array, so
   // it runs beside the flux pairs, before the barrier that phase 3 needs anyway
-  if constexpr (CONTRACT != 0) contract_mfma<NN, EPB, NT, CONTRACT>(sF, sA, sOut, tid, cdesc);
+  if constexpr (CONTRACT == 1 || (CONTRACT == 2 && !kLinesPerLane<NN>)) contract_mfma<NN, EPB, NT, CONTRACT>(sF, sA, sOut, tid);
+  if constexpr (CONTRACT == 2 && kLinesPerLane<NN>) contract_lines_bf16<NN, EPB, NT>(sF, sm.sAp, sOut, tid, ldesc);
   __syncthreads();
 
   if (!io) return;
@@ -1034,16 +1083,20 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
 }
 
 // Waves per SIMD the register allocation must allow. The FP64 stage needs more than the 80
-// registers a target of 6 leaves when the RK register is loaded early, or at 7 and 8 nodes per
-// line, and would spill to scratch; those shapes target 5 (96 registers).
-template <int NN, typename CT, int MINW, bool LATE>
+// registers a target of 6 leaves when the RK register is loaded early, and with late loads at
+// every order but 2 and 5 nodes per line, and would spill to scratch (8-20 bytes per lane); those
+// shapes target 5 (96 registers), and so does the split-BF16 stage at 2 and 3 nodes per line
+// (20-28 bytes at 6). No kernel of the library uses scratch.
+template <int NN, typename CT, int MINW, bool LATE, int CONTRACT = 0>
 constexpr int waves_target() {
-  return (MINW > 5 && sizeof(CT) == 8 && (!LATE || NN >= 7)) ? 5 : MINW;
+  constexpr bool fp64_heavy = sizeof(CT) == 8 && (!LATE || (NN != 2 && NN != 5));
+  constexpr bool bf16_heavy = CONTRACT == 2 && NN <= 3;
+  return (MINW > 5 && (fp64_heavy || bf16_heavy)) ? 5 : MINW;
 }
 
 template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT, int CONTRACT = 0, bool LATE = false,
           bool LDG = false>
-__global__ void __launch_bounds__((Geo<NN, EPB>::NT), (waves_target<NN, CT, MINW, LATE>())) k_rhs(
+__global__ void __launch_bounds__((Geo<NN, EPB>::NT), (waves_target<NN, CT, MINW, LATE, CONTRACT>())) k_rhs(
     const RhsLaunch L, const GroupMap gm) {
   __shared__ StageSmem<NN, EPB, CT, CONTRACT, LDG> sm;
   const uint32_t grp = map_group(blockIdx.x, gm);

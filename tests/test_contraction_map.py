@@ -1,5 +1,5 @@
-"""The matrix-core contraction's tile mapping (contract_mfma in 3ddg_amd/csrc/dg3d_kernels.hip),
-emulated on the CPU for every stage shape the library builds.
+"""The matrix-core contractions' mappings (contract_mfma and contract_lines_bf16 in
+3ddg_amd/csrc/dg3d_kernels.hip), emulated on the CPU for every stage shape the library builds.
 
 The emulation restates the kernel's index arithmetic: the column -> line map (column_line), the
 aligned diagonal blocks (one every 4 rows for n <= 4, every 8 rows for n >= 5), the lane layouts of
@@ -126,6 +126,7 @@ def test_tile_map_covers_every_line_once_in_bounds(n):
     for epb in SHAPES[n]:
         sF = rng.standard_normal(epb * 4 * n ** 3)
         ref = direct(n, epb, A, sF)
+        # the 16x16x16 split-BF16 tiles run at 8 nodes per line only, but the map holds for all
         for mode in (1, 2):
             got, writes = emulate(n, epb, mode, A, sF)
             assert (writes == 1).all(), (n, epb, mode, writes.min(), writes.max())
@@ -133,9 +134,9 @@ def test_tile_map_covers_every_line_once_in_bounds(n):
 
 
 def descriptor(n, epb, col):
-    """The split-BF16 contraction's column descriptor (ColumnTable in dg3d_kernels.hip): bits 0-11
-    the line's first node in sF, 12-24 in sOut, 25-26 the axis, 27 a P/rho0 line, 31 a column
-    that exists; 0 for the padding columns of the last tile."""
+    """The split-BF16 line descriptor (LineTable in dg3d_kernels.hip): bits 0-11 the line's first
+    node in sF, 12-24 in sOut, 25-26 the axis, 27 a P/rho0 line, 31 a line that exists; 0 for the
+    padding lines of the last group of 64."""
     NF2 = n * n
     if col >= epb * 6 * NF2:
         return 0
@@ -146,37 +147,74 @@ def descriptor(n, epb, col):
     return fo | oo << 12 | a << 25 | int(pressure) << 27 | 1 << 31
 
 
-@pytest.mark.parametrize("n", sorted(SHAPES))
-def test_column_descriptors_decode_to_the_column_lines(n):
-    """What the split-BF16 tile loop decodes from a lane's descriptors equals column_line's line
-    (first nodes, stride, shift flag) for every column of every tile a wave takes, and the padding
-    columns of the last tile decode as absent."""
-    NF2 = n * n
-    for epb in SHAPES[n]:
-        NT = (epb * elem_stride(n) + 63) // 64 * 64
-        W = NT // 64
-        BS = 4 if n <= 4 else 8
-        TC = 16 * (16 // BS)
-        C = epb * 6 * NF2
-        T = (C + TC - 1) // TC
-        TMAX = (T + W - 1) // W
-        seen = set()
-        for wave in range(W):
+def mfma_4x4x4(A_lanes, B_lanes):
+    """v_mfma_f32_4x4x4bf16_1k as probed on the device: 16 blocks of 4 lanes; lane 4b + i holds
+    row i of block b's A (4 K values), lane 4b + j column j of block b's B, and register r of lane
+    4b + j receives row r, column j of block b's product."""
+    D = np.zeros((64, 4))
+    for b in range(16):
+        Ab = A_lanes[4 * b:4 * b + 4]          # [i][k]
+        Bb = B_lanes[4 * b:4 * b + 4].T        # [k][j]
+        D[4 * b:4 * b + 4] = (Ab @ Bb).T       # lane j, register r
+    return D
+
+
+def emulate_lines(n, epb, A, sF):
+    """The line-per-lane split-BF16 contraction (contract_lines_bf16) in exact arithmetic: each
+    lane decodes its line from its descriptor, loads it (P/rho0 lines shifted by their middle
+    value), and takes its rows from the 16-block 4x4x4 products of the pieces of A."""
+    NP, NF2 = n ** 3, n * n
+    NT = (epb * elem_stride(n) + 63) // 64 * 64
+    W = NT // 64
+    C = epb * 6 * NF2
+    G = (C + 63) // 64
+    RB = KB = 2 if n > 4 else 1
+    Ap = np.zeros((4 * RB, 4 * KB))
+    Ap[:n, :n] = A
+    sOut = np.full(epb * 6 * NP, np.nan)
+    writes = np.zeros(epb * 6 * NP, int)
+    for wave in range(W):
+        for g in range(wave, G, W):
+            x = np.zeros((64, 4 * KB))
+            dec = []
             for lane in range(64):
-                bd = (4 * (lane >> 4)) // BS
-                for k in range(TMAX):
-                    t = wave + k * W
-                    if t >= T:
-                        break
-                    col = t * TC + 16 * bd + (lane & 15)
-                    d = descriptor(n, epb, col)
-                    ok = d >> 31 & 1
-                    assert ok == (col < C)
-                    if not ok:
-                        continue
-                    fo, oo, ax, pr = d & 0xFFF, d >> 12 & 0x1FFF, d >> 25 & 3, d >> 27 & 1
-                    ls = NF2 if ax == 0 else (n if ax == 1 else 1)
-                    assert (fo, oo, ls) == column_line(n, col)
-                    assert pr == int(((col % (6 * NF2)) // NF2) % 2 == 0)
-                    seen.add(col)
-        assert seen == set(range(C))
+                d = descriptor(n, epb, g * 64 + lane)
+                ok = d >> 31 & 1
+                fo = d & 0xFFF if ok else 0
+                oo, ax, pr = d >> 12 & 0x1FFF, d >> 25 & 3, d >> 27 & 1
+                ls = NF2 if ax == 0 else (n if ax == 1 else 1)
+                for m in range(n):
+                    assert 0 <= fo + m * ls < sF.size
+                    x[lane, m] = sF[fo + m * ls]
+                sh = x[lane, n // 2] if pr else 0.0
+                x[lane, :n] = (x[lane, :n] - sh) if ok else 0.0
+                dec.append((ok, oo, ls))
+            for R in range(RB):
+                out = np.zeros((64, 4))
+                for K in range(KB):
+                    A_lanes = np.array([Ap[4 * R + (lane & 3), 4 * K:4 * K + 4] for lane in range(64)])
+                    out += mfma_4x4x4(A_lanes, x[:, 4 * K:4 * K + 4])
+                for lane, (ok, oo, ls) in enumerate(dec):
+                    for r in range(4):
+                        row = 4 * R + r
+                        if ok and row < n:
+                            i = oo + row * ls
+                            assert 0 <= i < sOut.size
+                            sOut[i] = out[lane, r]
+                            writes[i] += 1
+    return sOut, writes
+
+
+@pytest.mark.parametrize("n", [k for k in sorted(SHAPES) if k < 8])
+def test_line_per_lane_map_covers_every_line_once_in_bounds(n):
+    """At 2..7 nodes per line the split-BF16 contraction takes one line per lane: every line is
+    written exactly once, every LDS index is in bounds, and the result equals the direct per-node
+    contraction; the pieces of A sit on a grid of 4x4 blocks (one for n <= 4, four above)."""
+    rng = np.random.default_rng(100 + n)
+    A = rng.standard_normal((n, n))
+    for epb in SHAPES[n]:
+        sF = rng.standard_normal(epb * 4 * n ** 3)
+        ref = direct(n, epb, A, sF)
+        got, writes = emulate_lines(n, epb, A, sF)
+        assert (writes == 1).all(), (n, epb, writes.min(), writes.max())
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
