@@ -1,7 +1,7 @@
 // C ABI of the batched path: many independent blocks ("solves") of different element counts and
-// orders, advanced by LSRK54. Per stage, every order present gets one persistent launch over all
-// its solves' element groups; the launches of a stage run as parallel graph branches, and whole
-// steps are captured in HIP graphs and replayed.
+// orders, advanced by LSRK54. Per stage, every order present gets one launch over all its solves'
+// element groups (or, under schedule 4, one persistent launch covers every order); the launches
+// of a stage run under a tuned schedule, and whole steps are captured in HIP graphs and replayed.
 //
 // Each solve is what one dg3d_ctx holds (Grid.cpp:176-375 builds one such block per rank): its
 // own mesh, material, operators and state. The per-node arithmetic is the fused stage of
@@ -78,6 +78,11 @@ struct dg3d_batch {
     int branch = -1;                   // stream index this order runs on (-1: the batch stream)
   };
   std::vector<Group> groups;
+  // schedule 4: every order's element groups in one task list (largest order first) for one
+  // persistent launch, and that launch's grid
+  dg3d::BatchTask* d_all_tasks = nullptr;
+  int all_ntasks = 0;
+  int all_grid = 1;
   bool tuned = false;
   int schedule = 0;  // how a stage runs the orders (kScheduleBranches, kScheduleSerial, kScheduleTwo)
   hipStream_t stream = nullptr;
@@ -125,6 +130,7 @@ void free_batch(dg3d_batch* b) {
   (void)hipFree(b->d_clocks);
   (void)hipFree(b->d_dts);
   (void)hipFree(b->d_offsets);
+  (void)hipFree(b->d_all_tasks);
   for (auto& g : b->groups) {
     for (auto* t : g.d_tasks) (void)hipFree(t);
     if (g.done) (void)hipEventDestroy(g.done);
@@ -150,10 +156,11 @@ dg3d::BatchLaunch stage_launch(const dg3d_batch* b, const dg3d_batch::Group& g, 
 
 // Stage schedules: every order on a branch of its own (the orders share the device); all orders
 // one after another on the batch stream, largest first (each has the device to itself); two
-// branches with about the same number of nodes; or the largest order on one branch and the
-// others one after another on a second, so that they run beside it and fill the device while its
-// last workgroups drain.
-constexpr int kScheduleBranches = 0, kScheduleSerial = 1, kScheduleTwo = 2, kScheduleBeside = 3, kNumSchedules = 4;
+// branches with about the same number of nodes; the largest order on one branch and the others
+// one after another on a second, so that they run beside it and fill the device while its last
+// workgroups drain; or every order in one persistent launch (k_rhs_batch_all).
+constexpr int kScheduleBranches = 0, kScheduleSerial = 1, kScheduleTwo = 2, kScheduleBeside = 3, kScheduleOne = 4,
+              kNumSchedules = 5;
 
 void set_schedule(dg3d_batch* b, int schedule) {
   b->schedule = schedule;
@@ -161,7 +168,7 @@ void set_schedule(dg3d_batch* b, int schedule) {
   for (size_t i = 0; i < b->groups.size(); ++i) {
     auto& g = b->groups[i];
     if (schedule == kScheduleBranches) g.branch = (int)i;
-    else if (schedule == kScheduleSerial) g.branch = -1;
+    else if (schedule == kScheduleSerial || schedule == kScheduleOne) g.branch = -1;
     else if (schedule == kScheduleBeside) g.branch = (i == 0) ? 0 : 1;  // groups are in descending order
     else if (schedule == kScheduleTwo) {
       // groups are in descending order; each goes to the branch with fewer nodes so far
@@ -180,6 +187,16 @@ hipError_t launch_stage(dg3d_batch* b, int stage, bool halo, int from) {
   // the source faces' exterior traces at every solve's stage time, before the stage kernels
   hipError_t e = dg3d::launch_batch_sources(b->d_sources, b->nsources, b->source_items, b->d_clocks, b->d_offsets,
                                             stage, halo ? 1 : 0, b->stream);
+  if (b->schedule == kScheduleOne) {
+    if (e != hipSuccess) return e;
+    dg3d::BatchLaunch B{};
+    B.records = b->d_records;
+    B.tasks = b->d_all_tasks;
+    B.ntasks = b->all_ntasks;
+    B.record = (halo && stage == 0) ? 10 + from : 2 * stage + from;
+    B.mixed = (b->precision == DG3D_MIXED) ? 1 : 0;
+    return dg3d::launch_rhs_batch_all(B, b->all_grid, b->stream);
+  }
   if (e == hipSuccess) e = hipEventRecord(b->fork, b->stream);
   for (size_t i = 0; i < ng && e == hipSuccess; ++i)
     if (used[i]) e = hipStreamWaitEvent(b->groups[i].stream, b->fork, 0);
@@ -555,6 +572,23 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
     if (e == hipSuccess) e = hipEventCreateWithFlags(&g.done, hipEventDisableTiming);
     b->groups.push_back(g);
     if (e != hipSuccess) return bail(DG3D_E_HIP, std::string("batch streams: ") + hipGetErrorString(e));
+  }
+  {
+    // schedule 4's task list: every order's element groups, largest order first
+    std::vector<dg3d::BatchTask> tasks;
+    for (int n : orders) {
+      const int epb = dg3d::batch_all_epb(n);
+      for (int i = 0; i < count; ++i) {
+        if (b->solves[i].n != n) continue;
+        for (int64_t e0 = 0; e0 < b->solves[i].E; e0 += epb) tasks.push_back(dg3d::BatchTask{i, (int)e0});
+      }
+    }
+    if (tasks.size() >= (size_t)INT32_MAX) return bail(DG3D_E_INVALID, "batch: too many element groups");
+    if (bmalloc(&b->d_all_tasks, tasks.size()) != hipSuccess) return bail(DG3D_E_NOMEM, "batch: task list allocation failed");
+    upload(hipMemcpy(b->d_all_tasks, tasks.data(), sizeof(dg3d::BatchTask) * tasks.size(), hipMemcpyHostToDevice));
+    b->all_ntasks = (int)tasks.size();
+    const int resident = dg3d::batch_all_max_grid(precision == DG3D_MIXED ? 1 : 0);
+    b->all_grid = std::max(1, std::min(b->all_ntasks, resident > 0 ? resident : 1024));
   }
   set_schedule(b, kScheduleBranches);
   e = hipDeviceSynchronize();

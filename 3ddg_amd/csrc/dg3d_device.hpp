@@ -258,6 +258,11 @@ int batch_epb(int variant, int n);
 // One persistent launch over the tasks of the solves of order n-1 (B.tasks holds only those).
 hipError_t launch_rhs_batch(const BatchLaunch& B, int n, int variant, int grid, hipStream_t s);
 int batch_max_grid(int n, int variant, int mixed);  // resident workgroups on the device
+// Every order in one persistent launch (batch schedule 4): B.tasks holds the element groups of all
+// solves, each group batch_all_epb(n) elements of its solve's order n.
+int batch_all_epb(int n);
+hipError_t launch_rhs_batch_all(const BatchLaunch& B, int grid, hipStream_t s);
+int batch_all_max_grid(int mixed);  // resident workgroups of that launch on the device
 
 // Exterior trace of the source face at the stage time (x.t, or clock[0] + x.t) into `out`
 // (halo trace + boundary source), [faceElem][n*n][3]; the RHS kernel then reads it as that face's

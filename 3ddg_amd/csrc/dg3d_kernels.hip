@@ -1162,6 +1162,59 @@ __global__ void __launch_bounds__((Geo<NN, EPB>::NT)) k_rhs_batch(const BatchLau
 }
 
 #ifndef DG3D_TU_ORDER
+// Every order of a batch in one persistent launch (batch schedule 4): 512-lane workgroups take the
+// element groups of all solves from one task list (largest order first, per-XCD slabs) and run the
+// stage body of the task's order, with that order's elements per workgroup (a 512-lane shape
+// where one exists: 16, 8, 4 elements at 3, 4, 5 nodes per line, 32 at 2 and 2 at 6 leave half and
+// an eighth of the lanes idle, 1 at 7 and 8). The LDS is the union of the orders' layouts and the
+// registers the most any order needs, so every order runs at the occupancy of the heaviest.
+template <int NN>
+constexpr int all_epb() {
+  return NN == 2 ? 32 : NN == 3 ? 16 : NN == 4 ? 8 : NN == 5 ? 4 : NN == 6 ? 2 : 1;
+}
+constexpr int kAllNT = 512;
+template <typename CT>
+union AllSmem {
+  StageSmem<2, all_epb<2>(), CT, 0> s2;
+  StageSmem<3, all_epb<3>(), CT, 0> s3;
+  StageSmem<4, all_epb<4>(), CT, 0> s4;
+  StageSmem<5, all_epb<5>(), CT, 0> s5;
+  StageSmem<6, all_epb<6>(), CT, 0> s6;
+  StageSmem<7, all_epb<7>(), CT, 0> s7;
+  StageSmem<8, all_epb<8>(), CT, 0> s8;
+};
+
+template <typename CT, typename DUT>
+__global__ void __launch_bounds__(kAllNT) k_rhs_batch_all(const BatchLaunch B) {
+  __shared__ AllSmem<CT> sm;
+  const ConstTaskPtr tasks = (ConstTaskPtr)B.tasks;
+  const uint32_t w = blockIdx.x, x = w & 7u;
+  // fewer than 8 workgroups leave some XCD without one: then the plain order
+  uint32_t t0 = w, t1 = (uint32_t)B.ntasks, step = gridDim.x;
+  if (gridDim.x >= 8) {
+    xcd_slab((uint32_t)B.ntasks, x, &t0, &t1);
+    step = (gridDim.x - x + 7) >> 3;
+    t0 += w >> 3;
+  }
+  for (uint32_t t = t0; t < t1; t += step) {
+    __syncthreads();  // the previous task's phase 3 still reads LDS
+    int tid;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"((int)threadIdx.x));
+    const int solve = tasks[t].solve, e0 = tasks[t].e0;
+    const RhsLaunch& L = *(const RhsLaunch*)((ConstLaunchPtr)(B.records + (int64_t)solve * kBatchRecords + B.record));
+    switch (L.n) {  // the same for every lane of the workgroup
+      case 2: stage_body<2, all_epb<2>(), kAllNT, CT, EPI_LSRK, DUT, 0>(L, L.mesh, L.ext.trace, e0, tid, sm.s2); break;
+      case 3: stage_body<3, all_epb<3>(), kAllNT, CT, EPI_LSRK, DUT, 0>(L, L.mesh, L.ext.trace, e0, tid, sm.s3); break;
+      case 4: stage_body<4, all_epb<4>(), kAllNT, CT, EPI_LSRK, DUT, 0>(L, L.mesh, L.ext.trace, e0, tid, sm.s4); break;
+      case 5: stage_body<5, all_epb<5>(), kAllNT, CT, EPI_LSRK, DUT, 0>(L, L.mesh, L.ext.trace, e0, tid, sm.s5); break;
+      case 6: stage_body<6, all_epb<6>(), kAllNT, CT, EPI_LSRK, DUT, 0>(L, L.mesh, L.ext.trace, e0, tid, sm.s6); break;
+      case 7: stage_body<7, all_epb<7>(), kAllNT, CT, EPI_LSRK, DUT, 0>(L, L.mesh, L.ext.trace, e0, tid, sm.s7); break;
+      case 8: stage_body<8, all_epb<8>(), kAllNT, CT, EPI_LSRK, DUT, 0>(L, L.mesh, L.ext.trace, e0, tid, sm.s8); break;
+      default: break;
+    }
+  }
+}
+
 // Exterior trace of the source face at one stage time: the face's halo trace (or zero) plus the
 // boundary source, p = s(t + lead), rho' = s/c^2, u_axis = +-s/(rho0 c) entering the block.
 // Node i of the source face's trace at stage time ts.
@@ -1804,6 +1857,38 @@ int batch_max_grid(int n, int variant, int mixed) {
     case 8: return resident_o8(variant, mixed);
     default: return 0;
   }
+}
+
+int batch_all_epb(int n) {
+  switch (n) {
+    case 2: return all_epb<2>();
+    case 3: return all_epb<3>();
+    case 4: return all_epb<4>();
+    case 5: return all_epb<5>();
+    case 6: return all_epb<6>();
+    case 7: return all_epb<7>();
+    case 8: return all_epb<8>();
+    default: return 0;
+  }
+}
+
+hipError_t launch_rhs_batch_all(const BatchLaunch& B, int grid, hipStream_t s) {
+  if (B.ntasks <= 0) return hipSuccess;
+  if (grid < 1) grid = 1;
+  if (B.mixed) hipLaunchKernelGGL((k_rhs_batch_all<float, float>), dim3(grid), dim3(kAllNT), 0, s, B);
+  else hipLaunchKernelGGL((k_rhs_batch_all<double, double>), dim3(grid), dim3(kAllNT), 0, s, B);
+  return hipGetLastError();
+}
+
+int batch_all_max_grid(int mixed) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  const hipError_t e =
+      mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch_all<float, float>, kAllNT, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch_all<double, double>, kAllNT, 0);
+  if (e != hipSuccess || per < 1) per = 1;
+  return cus * per;
 }
 
 int batch_epb(int variant, int n) {

@@ -265,8 +265,10 @@ int dg3d_batch_set_variant(dg3d_batch* batch, int variant);
 /* How a stage runs the orders: 0 every order on a parallel branch of its own, 1 all orders one
  * after another on one stream (largest first, each with the device to itself), 2 two branches
  * with about the same number of nodes, 3 the largest order on one branch and the others one after
- * another on a second beside it. dg3d_batch_autotune (and the first dg3d_batch_step) times the
- * four on whole stages after choosing the variants; results do not depend on it. */
+ * another on a second beside it, 4 every order in one persistent launch (one kernel whose
+ * workgroups take element groups of any order). dg3d_batch_autotune (and the first
+ * dg3d_batch_step) times the five on whole stages after choosing the variants; results do not
+ * depend on it. */
 int dg3d_batch_set_schedule(dg3d_batch* batch, int schedule);
 int dg3d_batch_get_schedule(const dg3d_batch* batch, int* schedule);
 /* nsteps LSRK54 steps of every solve, each with its own dt; synchronises. */

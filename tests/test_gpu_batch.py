@@ -110,8 +110,8 @@ def test_batch_mixed_vs_fp64():
 def test_batch_shapes_bitwise_identical(precision):
     """Every variant (two elements-per-workgroup shapes, persistent or one group per workgroup,
     early or late RK-register loads) and every stage schedule (parallel branches per order, one
-    stream, two balanced branches, the largest order beside the others) computes the same
-    per-node arithmetic."""
+    stream, two balanced branches, the largest order beside the others, every order in one
+    persistent launch) computes the same per-node arithmetic."""
     print(f"batch_shapes precision={precision}", flush=True)
     ps = mixed_set()
     states = []
@@ -124,7 +124,7 @@ def test_batch_shapes_bitwise_identical(precision):
     for v in range(1, 8):
         for i in range(len(ps)):
             assert np.array_equal(states[0][i], states[v][i]), (v, i)
-    for sc in range(4):
+    for sc in range(5):
         b = problem.make_batch(ps, precision=precision)
         b.set_variant(5)
         b.set_schedule(sc)
@@ -136,7 +136,7 @@ def test_batch_shapes_bitwise_identical(precision):
     b = problem.make_batch(ps, precision=precision)
     v = b.autotune()
     assert len(v) == b.info()["num_orders"] and all(0 <= x < 8 for x in v)
-    assert 0 <= b.schedule() < 4
+    assert 0 <= b.schedule() < 5
     b.close()
 
 
