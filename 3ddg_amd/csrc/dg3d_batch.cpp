@@ -510,7 +510,7 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
       if (src_out) L.ext.trace[d.source_face] = src_out;
       recs[(size_t)i * dg3d::kBatchRecords + r] = L;
     }
-    // initial state: host [4][E][np] -> element-blocked slots (dg3d::reg_slot), padding zero
+    // initial state: host [4][E][np] -> group-blocked slots (dg3d::reg_slot), padding zero
     stage.assign((size_t)4 * h.fs, 0.0);
     for (int f = 0; f < 4; ++f)
       for (int64_t el = 0; el < h.E; ++el)

@@ -657,7 +657,7 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
   if (d->order >= 1 && d->order <= 7 &&
       dg3d::field_slots((int64_t)d->num_elements[0] * d->num_elements[1] * d->num_elements[2], d->order + 1) >=
           (1ll << 29)) {
-    // byte offsets within one field array are 32-bit in the kernels
+    // the reductions and the probe index a register's state fields with 32-bit slot numbers
     g_create_error = "2^29 or more nodes in one block: split the domain into blocks (NUM_BLOCKS)";
     return DG3D_E_INVALID;
   }

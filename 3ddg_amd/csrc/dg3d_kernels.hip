@@ -125,10 +125,10 @@ __device__ __forceinline__ int face_node(int axis, int ia, int b, int c) {
   return (b * NN + c) * NN + ia;
 }
 
-// Buffer-resource access. One resource per field array (base, size in bytes): the loads and
-// stores of all fields then share one 32-bit offset register, and an access beyond the array
-// reads 0 / is dropped. The context guarantees fewer than 2^29 nodes per field, so byte
-// offsets fit in 32 bits.
+// Buffer-resource access. A stage workgroup's resources start at its first element's slot and
+// span its elements' pieces of the register (base, size in bytes): the loads and stores of all
+// fields then share one small 32-bit offset register, and an access beyond the range reads 0 /
+// is dropped.
 using Rsrc = __amdgpu_buffer_rsrc_t;
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 constexpr int kPolicyNT = 2;  // nontemporal

@@ -91,7 +91,7 @@ STREAM_PROBE = os.path.join(ROOT, "3ddg_amd", "dg3d_stream_probe")
 
 def stream_ceiling():
     """The device's streaming rate for the LSRK stage kernel's access pattern at the 64^3 N=4
-    block's size (element-blocked FP64 state read and written, pair-major RK register read and
+    block's size (group-blocked FP64 state read and written, pair-major RK register read and
     written in place, workgroups over whole elements), from 3ddg_amd/dg3d_stream_probe run as a
     child process. None when the probe is not built or fails."""
     if not os.path.exists(STREAM_PROBE):

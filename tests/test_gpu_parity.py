@@ -417,8 +417,9 @@ def _smooth_random_state(p, seed, rho_scale=0.3, u_scale=0.3):
 @pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
 @pytest.mark.parametrize("order,n_e", [(7, 3), (1, 5)])
 def test_fused_orders_1_and_7_vs_oracle(order, n_e, scheme, precision):
-    """The fused step at 8 and 2 nodes per line, where the LSRK register has its own layouts
-    (field-major at 8 nodes, pair-major FP64 / node-major FP32 at 2) and the stage its own launch
+    """The fused step at 8 and 2 nodes per line, where the LSRK register (pair-major FP64,
+    node-major FP32) and the group-blocked state have their largest and smallest groups of
+    elements (8 and 512) and the stage its own launch
     bounds: heterogeneous c and rho0 with B/A = 5 everywhere, random state on every node, the tone
     burst at its peak (the clock starts 30 steps before the envelope's centre), 60 steps. Every
     variant of the stage kernel runs; 27 elements at N = 7 leave a partial last workgroup under
@@ -552,8 +553,8 @@ def test_ldg_q_kernel_orders_vs_oracle(order):
 @pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
 @pytest.mark.parametrize("order", [1, 4])
 def test_register_slots_past_the_elements_stay_zero(scheme, order):
-    """The slots between the last element and the next field (the field pitch is skewed away from
-    powers of two) are never written by a stage, yet the reductions cover them. A zero state with
+    """The slots past a group's elements in each field run, and those of a last, partial group,
+    are never written by a stage, yet the reductions cover them. A zero state with
     no source, autotuned (the tuner streams a pseudo-random state through the work registers)
     and stepped (LSRK54 hands the state over by pointer, so register 0 becomes a work register):
     the max |.| over every slot stays exactly zero."""
