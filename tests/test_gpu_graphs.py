@@ -6,7 +6,7 @@ precisions, and graphs dropped by a variant change and by a new dt."""
 import numpy as np
 import pytest
 
-from helpers import dg3d, heterogeneous, problem, small_c1, small_c2, small_c5
+from helpers import dg3d, heterogeneous, problem, random_state, small_c1, small_c2, small_c5
 
 pytestmark = pytest.mark.gpu
 
@@ -73,3 +73,34 @@ def test_graphs_ldg_diffusion(scheme):
     p = heterogeneous(small_c2(n_e=3, order=2))
     p.diffusion = np.full(p.E, 1.0e-3)
     _check(p, scheme, dg3d.FP64, with_q=True)
+
+
+@pytest.mark.parametrize("order,n_e", [(5, 5), (1, 9), (6, 5)])
+def test_probe_rows_are_the_pressure_at_the_probe_nodes(order, n_e):
+    """Each probe row holds P = c^2 (rho' + rho'^2 (B/A) / (2 rho0)) at the probe nodes (host
+    numbering) after a step: checked against the state read back after every single step, on meshes
+    whose last group of elements is partial (125 elements in groups of 18 and 10 at 6 and 7 nodes per
+    line, 729 in groups of 512 at 2), with heterogeneous material and random state."""
+    print(f"probe_rows order={order} n_e={n_e}", flush=True)
+    p = heterogeneous(small_c2(n_e=n_e, order=order), seed=7)
+    rng = np.random.default_rng(3)
+    nodes = np.unique(np.concatenate([rng.integers(0, p.dof, size=40), [0, p.dof - 1]])).astype(np.int64)
+    ctx = problem.make_context(p)
+    ctx.set_state(0, random_state(p, seed=4))
+    ctx.set_probes(nodes, 8)
+    dt = problem.cfl_dt_host(p)
+    t = 0.0
+    expect = []
+    np3 = p.n ** 3
+    e = nodes // np3
+    for _ in range(3):
+        t = ctx.step(dg3d.LSRK54, dt, 1, t)
+        rho = ctx.get_state(0)[: p.dof][nodes]
+        c, r0, ba = p.c0[e], p.rho0[e], p.b_over_a[e]
+        expect.append(c * c * (rho + ba * 0.5 / r0 * rho * rho))
+    got = ctx.get_probes()
+    ctx.close()
+    assert got.shape == (3, nodes.size)
+    expect = np.asarray(expect)
+    assert np.abs(expect).max() > 0
+    np.testing.assert_allclose(got, expect, rtol=1e-13, atol=1e-13 * np.abs(expect).max())
