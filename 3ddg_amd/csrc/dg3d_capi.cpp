@@ -78,7 +78,7 @@ struct dg3d_ctx {
   double* d_wq = nullptr;
   dg3d_source src{};
   double* d_lead = nullptr;
-  double* d_src_trace = nullptr;  // exterior trace of the source face at the current stage time
+  double* d_src_trace = nullptr;  // exterior traces of the source face, one slot per stage of a step
   double* d_clock = nullptr;    // dg3d_step's time at the start of the current step
   int* d_probe_step = nullptr;  // probe row the next step writes
   bool use_graphs = true;
@@ -156,14 +156,17 @@ struct dg3d_ctx {
   bool reg_ok(int r) const { return r >= 0 && r < (int)regs.size(); }
   // Exterior arguments of the RHS kernel: with a source, a small kernel first writes the source
   // face's trace (halo + source at time t) and the RHS kernel reads it as that face's halo.
-  // With dev_clock, t is the stage's offset from the device clock of dg3d_step.
-  hipError_t ext_rhs(int reg, double t, dg3d::ExtArgs* out, bool dev_clock = false) const {
+  // With dev_clock, t is the stage's offset from the device clock of dg3d_step. With src_slot >= 0
+  // the step has already written the trace of this stage into that slot (launch_step).
+  hipError_t ext_rhs(int reg, double t, dg3d::ExtArgs* out, bool dev_clock = false, int src_slot = -1) const {
     dg3d::ExtArgs x = ext_for(reg, t);
     if (dev_clock) x.clock = d_clock;
     if (src.face >= 0) {
-      hipError_t e = dg3d::launch_source_trace(x, mesh(), n, x.trace[src.face], d_src_trace, stream);
-      if (e != hipSuccess) return e;
-      x.trace[src.face] = d_src_trace;
+      if (src_slot < 0) {
+        hipError_t e = dg3d::launch_source_trace(x, mesh(), n, x.trace[src.face], d_src_trace, stream);
+        if (e != hipSuccess) return e;
+      }
+      x.trace[src.face] = d_src_trace + (src_slot > 0 ? src_slot : 0) * dg3d::source_trace_count(src.face, mesh(), n) * 3;
       x.src_face = -1;
     }
     *out = x;
@@ -322,10 +325,18 @@ dg3d::RhsLaunch base_launch(const dg3d_ctx* c) {
   return L;
 }
 
+// Time of a stage of one step from the step's start: LSRK54's c_s dt; RK33's 0, dt, dt/2 (the
+// times of Grid.cpp:705, 765-766, 826-827)
+double stage_offset(int scheme, int stage, double dt) {
+  if (scheme == DG3D_LSRK54) return kLsC[stage] * dt;
+  return stage == 0 ? 0.0 : (stage == 1 ? dt : 0.5 * dt);
+}
+
 // One fused stage. With dev_clock, t is 0 and the source trace reads the step's start time from
-// the device clock, so that the launches do not depend on the step being run.
+// the device clock, so that the launches do not depend on the step being run. With src_slot >= 0
+// the source trace of the stage is already in that slot of the trace buffer.
 int run_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t, double** cur, double** other, bool first,
-              bool dev_clock = false) {
+              bool dev_clock = false, int src_slot = -1) {
   dg3d::RhsLaunch L = base_launch(c);
   if (c->tune_variant[scheme] >= 0) {
     L.variant = c->tune_variant[scheme];
@@ -341,7 +352,7 @@ int run_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t, double** 
     L.du = c->work_du;
     L.a = kLsA[stage];
     L.b = kLsB[stage];
-    hipError_t e = c->ext_rhs(first ? 0 : -1, t + kLsC[stage] * dt, &L.ext, dev_clock);
+    hipError_t e = c->ext_rhs(first ? 0 : -1, t + stage_offset(scheme, stage, dt), &L.ext, dev_clock, src_slot);
     if (e != hipSuccess) return c->hip_fail(e, "source trace");
   } else {
     L.epilogue = dg3d::EPI_RK33;
@@ -349,14 +360,12 @@ int run_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t, double** 
     hipError_t e;
     if (stage == 0) {
       L.q_in = u0; L.out = c->work_k1; L.old = nullptr; L.a = 0.0; L.b = 1.0;
-      e = c->ext_rhs(first ? 0 : -1, t, &L.ext, dev_clock);
     } else if (stage == 1) {
       L.q_in = c->work_k1; L.out = c->work_k2; L.old = u0; L.a = 0.75; L.b = 0.25;
-      e = c->ext_rhs(-1, t + dt, &L.ext, dev_clock);
     } else {
       L.q_in = c->work_k2; L.out = u0; L.old = u0; L.a = 1.0 / 3.0; L.b = 2.0 / 3.0;
-      e = c->ext_rhs(-1, t + 0.5 * dt, &L.ext, dev_clock);
     }
+    e = c->ext_rhs(first && stage == 0 ? 0 : -1, t + stage_offset(scheme, stage, dt), &L.ext, dev_clock, src_slot);
     if (e != hipSuccess) return c->hip_fail(e, "source trace");
   }
   if (c->has_diff && c->nfields == 7) {
@@ -379,12 +388,22 @@ int run_stage(dg3d_ctx* c, int scheme, int stage, double dt, double t, double** 
   return DG3D_OK;
 }
 
-// The launches of one step of dg3d_step: the stages, then the probe row and the device clock.
-// Only the first stage of a call may read register 0's halo traces (`first`).
+// The launches of one step of dg3d_step: the source traces of all its stages (one launch), the
+// stages, then the probe row and the device clock. Only the first stage of a call may read
+// register 0's halo traces (`first`).
 int launch_step(dg3d_ctx* c, int scheme, double dt, double** cur, double** other, bool first) {
   const int ns = (scheme == DG3D_LSRK54) ? 5 : 3;
+  if (c->src.face >= 0) {
+    dg3d::ExtArgs x = c->ext_for(first ? 0 : -1, 0.0);
+    x.clock = c->d_clock;
+    dg3d::StageTimes offs{};
+    for (int st = 0; st < ns; ++st) offs.v[st] = stage_offset(scheme, st, dt);
+    const hipError_t e =
+        dg3d::launch_source_trace_step(x, c->mesh(), c->n, x.trace[c->src.face], c->d_src_trace, offs, ns, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "source trace");
+  }
   for (int st = 0; st < ns; ++st) {
-    const int rc = run_stage(c, scheme, st, dt, 0.0, cur, other, first && st == 0, true);
+    const int rc = run_stage(c, scheme, st, dt, 0.0, cur, other, first && st == 0, true, c->src.face >= 0 ? st : -1);
     if (rc) return rc;
   }
   hipError_t e = hipSuccess;
@@ -939,7 +958,7 @@ int dg3d_set_source(dg3d_ctx* c, const dg3d_source* s) {
   if (!(s->tau > 0.0)) return c->fail(DG3D_E_INVALID, "source tau must be positive");
   c->src = *s;
   c->src.lead = nullptr;
-  if (dmalloc(&c->d_src_trace, (size_t)c->face_elems(s->face) * c->n * c->n * 3) != hipSuccess)
+  if (dmalloc(&c->d_src_trace, (size_t)c->face_elems(s->face) * c->n * c->n * 3 * dg3d::kMaxStages) != hipSuccess)
     return c->fail(DG3D_E_NOMEM, "source allocation failed");
   if (s->lead) {
     const size_t cnt = (size_t)c->face_elems(s->face) * c->n * c->n;

@@ -270,6 +270,24 @@ int batch_all_max_grid(int mixed);  // resident workgroups of that launch on the
 hipError_t launch_source_trace(const ExtArgs& x, const MeshArgs& m, int n, const double* halo, double* out,
                                hipStream_t s);
 
+// Values per stage of the source face's trace: [faceElem][n*n][3].
+inline int64_t source_trace_count(int face, const MeshArgs& m, int n) {
+  const int axis = 2 - face / 2;
+  const int64_t nb = (axis == 0) ? m.nE[1] : m.nE[0];
+  const int64_t nc = (axis == 2) ? m.nE[1] : m.nE[2];
+  return nb * nc * n * n;
+}
+constexpr int kMaxStages = 5;
+struct StageTimes {
+  double v[kMaxStages];
+};
+// The source face's traces of every stage of one step in one launch: stage s at the step's start
+// on the device clock (x.clock; absolute times without one) plus offs.v[s], into
+// out + s * source_trace_count * 3. Stage 0 adds halo0 (the halo traces a call's first stage
+// reads, or null); the later stages have no halo.
+hipError_t launch_source_trace_step(const ExtArgs& x, const MeshArgs& m, int n, const double* halo0, double* out,
+                                    const StageTimes& offs, int stages, hipStream_t s);
+
 // Boundary sources of the batched path: one entry per solve with a source. A stage's launch
 // computes every entry's exterior trace at the solve's stage time clocks[solve] +
 // offsets[solve * 5 + stage] (the offsets C_s dt prepared on the host, as dg3d_step's).

@@ -1249,6 +1249,17 @@ __global__ void k_source_trace(ExtArgs x, MeshArgs m, int n, const double* halo,
   source_trace_node(x, m, n, halo, out, ts, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// The source traces of every stage of one step in one launch (blockIdx.y = stage s): stage s at
+// the step's start on the device clock (or 0 without one) plus offs.v[s], into out + s count 3.
+// Only stage 0 adds `halo0`, the halo traces the first stage of a call reads.
+__global__ void k_source_trace_step(ExtArgs x, MeshArgs m, int n, const double* halo0, double* out,
+                                    StageTimes offs, int64_t count) {
+  const int s = (int)blockIdx.y;
+  const double ts = (x.clock ? x.clock[0] : 0.0) + offs.v[s];
+  source_trace_node(x, m, n, s == 0 ? halo0 : nullptr, out + (int64_t)s * count * 3, ts,
+                    (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Every batched solve's source trace for one stage: blockIdx.y picks the entry.
 __global__ void k_batch_source(const BatchSource* list, const double* clocks, const double* offsets, int stage,
                                int first) {
@@ -1914,6 +1925,16 @@ hipError_t launch_source_trace(const ExtArgs& x, const MeshArgs& m, int n, const
   const int64_t nc = (axis == 2) ? m.nE[1] : m.nE[2];
   const int64_t count = nb * nc * n * n;
   hipLaunchKernelGGL(k_source_trace, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, x, m, n, halo, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_source_trace_step(const ExtArgs& x, const MeshArgs& m, int n, const double* halo0, double* out,
+                                    const StageTimes& offs, int stages, hipStream_t s) {
+  if (x.src_face < 0) return hipSuccess;
+  if (stages < 1 || stages > kMaxStages) return hipErrorInvalidValue;
+  const int64_t count = source_trace_count(x.src_face, m, n);
+  hipLaunchKernelGGL(k_source_trace_step, dim3((unsigned)((count + 255) / 256), (unsigned)stages), dim3(256), 0, s, x,
+                     m, n, halo0, out, offs, count);
   return hipGetLastError();
 }
 
