@@ -104,3 +104,36 @@ def test_probe_rows_are_the_pressure_at_the_probe_nodes(order, n_e):
     expect = np.asarray(expect)
     assert np.abs(expect).max() > 0
     np.testing.assert_allclose(got, expect, rtol=1e-13, atol=1e-13 * np.abs(expect).max())
+
+
+@pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
+def test_step_calls_equal_stage_calls_with_source(scheme):
+    """dg3d_step writes the source face's traces of all stages of a step in one launch at the
+    step's start, from the device clock; dg3d_stage writes one stage's trace just before that
+    stage, at the host's stage time. The times are the same FP64 sums, so the states agree bit
+    for bit, with graphs and with direct launches."""
+    print(f"step_calls_equal_stage_calls scheme={scheme}", flush=True)
+    p = heterogeneous(small_c2(n_e=4, order=3))
+    dt = problem.cfl_dt_host(p)
+    ns = 5 if scheme == dg3d.LSRK54 else 3
+    states = []
+    for graphs in (True, False):
+        ctx = problem.make_context(p, precision=dg3d.FP64)
+        ctx.set_variant(scheme, 2, 1)
+        ctx.set_graphs(graphs)
+        t = ctx.step(scheme, dt, 7, 0.0)
+        states.append((ctx.get_state(0), t))
+        ctx.close()
+    ctx = problem.make_context(p, precision=dg3d.FP64)
+    ctx.set_variant(scheme, 2, 1)
+    t = 0.0
+    for _ in range(7):
+        for st in range(ns):
+            ctx.stage(scheme, st, dt, t)
+        t = t + dt
+    ref = ctx.get_state(0)
+    ctx.close()
+    assert np.linalg.norm(ref) > 0
+    for s, ts in states:
+        assert ts == t
+        assert np.array_equal(s, ref), np.abs(s - ref).max()
