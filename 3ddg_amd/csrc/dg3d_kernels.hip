@@ -1185,8 +1185,11 @@ union AllSmem {
   StageSmem<8, all_epb<8>(), CT, 0> s8;
 };
 
+// Two 512-lane workgroups per CU (4 waves per SIMD, 128 registers): without the bound the FP64
+// kernel took 152 registers, so one workgroup fitted per CU and the stage ran 1.5x slower; at 6
+// waves per SIMD both precisions spill 200-320 bytes.
 template <typename CT, typename DUT>
-__global__ void __launch_bounds__(kAllNT) k_rhs_batch_all(const BatchLaunch B) {
+__global__ void __launch_bounds__(kAllNT, 4) k_rhs_batch_all(const BatchLaunch B) {
   __shared__ AllSmem<CT> sm;
   const ConstTaskPtr tasks = (ConstTaskPtr)B.tasks;
   const uint32_t w = blockIdx.x, x = w & 7u;
