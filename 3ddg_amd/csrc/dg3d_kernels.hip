@@ -310,7 +310,7 @@ struct TileLine {
 };
 
 // Orders whose split-BF16 contraction takes one line per lane (contract_lines_bf16); at 8 nodes per
-// line the 16x16x16 tiles of contract_mfma, whose two 8-row blocks fill each tile, ran 1.7 %
+// line the 16x16x16 tiles of contract_mfma_bf16, whose two 8-row blocks fill each tile, ran 1.7 %
 // faster (64^3 N=7), while at 5 and 6 nodes per line the lines ran 2.8 and 2.4 % faster than
 // those tiles (64^3 N=4, 48^3 N=5).
 template <int NN>
@@ -438,66 +438,79 @@ __device__ __forceinline__ void contract_lines_bf16(const float* sF, const bf16x
   }
 }
 
-// The workgroup's derivative contractions as one GEMM on the matrix cores:
+// The FP32 derivative contraction on the matrix cores, one line per lane: out = A line for every
+// line of the workgroup (P/rho0 lines entered shifted by their middle value, as in the VALU
+// contraction of the FP32 stage). v_mfma_f32_4x4x1_16b_f32 computes 16 independent 4x1 by 1x4
+// products; its lane layout, checked on the device before use: lane 4b + i holds row i of block
+// b's A operand, lane 4b + j column j of its B operand, and register r of lane 4b + j receives
+// row r, column j of block b's result. Every block takes the same column of A, so each lane's B
+// value is node k of its own line and its results are rows of its own line: NN products per row
+// block of 4, accumulated in FP32 in node order, as the VALU contraction sums them.
+template <int NN, int EPB, int NT>
+__device__ __forceinline__ void contract_lines_f32(const float* sF, const float* sA, float* sOut, int tid,
+                                                   const uint32_t* ldesc) {
+  using M = LineShape<NN, EPB, NT>;
+  constexpr int RB = (NN + 3) / 4;
+  const int wave = tid >> 6, ia = tid & 3;
+  // the operator entries this lane feeds: rows 4R + ia (zero past the line), every column
+  float a[RB][NN];
+#pragma unroll
+  for (int R = 0; R < RB; ++R)
+#pragma unroll
+    for (int k = 0; k < NN; ++k) a[R][k] = (4 * R + ia < NN) ? sA[(4 * R + ia) * NN + k] : 0.f;
+#pragma unroll
+  for (int g = 0; g < M::GMAX; ++g) {
+    if (wave + g * M::W >= M::G) break;  // wave-uniform
+    const uint32_t d = ldesc[g];
+    const bool ok = (d >> 31) != 0u;
+    const int fo = ok ? (int)(d & 0xFFFu) : 0, oo = (int)((d >> 12) & 0x1FFFu);
+    const uint32_t ax = (d >> 25) & 3u;
+    const int ls = ax == 0u ? NN * NN : (ax == 1u ? NN : 1);
+    float x[NN];
+#pragma unroll
+    for (int m = 0; m < NN; ++m) x[m] = sF[fo + m * ls];
+    const float sh = ((d >> 27) & 1u) ? x[NN / 2] : 0.f;
+#pragma unroll
+    for (int m = 0; m < NN; ++m) x[m] = ok ? x[m] - sh : 0.f;
+#pragma unroll
+    for (int R = 0; R < RB; ++R) {
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < NN; ++k) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(a[R][k], x[k], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (ok && 4 * R + r < NN) sOut[oo + (4 * R + r) * ls] = acc[r];
+    }
+  }
+}
+
+// The split-BF16 derivative contractions of a workgroup as one GEMM on the matrix cores, used at 8
+// nodes per line (other orders take contract_lines_bf16):
 // out[i][col] = sum_m A[i][m] line_col[m], one column per (element, axis, field, face node);
 // P/rho0 lines enter shifted by their middle value, as in the VALU contraction of the FP32 stage.
-// P copies of A sit block-diagonally in the 16x16 operand, block b on rows and K indices
-// [b*BS, b*BS + NN); the block stride BS (4 for NN <= 4, else 8) is a multiple of 4, so the 4
-// rows a lane holds (4g..4g+3, in both the K operand and the result) lie in one block. A tile
-// covers 16 columns per block (columns col0 + 16 b), and waves take tiles round-robin.
-//   CONTRACT 1: v_mfma_f32_16x16x4f32, FP32 operands, 4 K steps.
-//   CONTRACT 2 (8 nodes per line; other orders take contract_lines_bf16): v_mfma_f32_16x16x16bf16_1k
-//               with A and the lines split into three bf16 pieces each; the five products of
-//               combined order 1 and 2 are accumulated in FP32, smallest first, and the hi x hi
-//               products from zero, then added to them.
-// Lane layouts (16x16 tile, lane l, g = l/16, j = l%16):
-//   16x16x4f32:        A[j][k = g], B[k = g][j], D[4g + r][j]   (K step s covers 4s..4s+3)
-//   16x16x16bf16_1k:   A[j][k = 4g + t], B[k = 4g + t][j], D[4g + r][j]
-template <int NN, int EPB, int NT, int CONTRACT>
-__device__ __forceinline__ void contract_mfma(const float* sF, const float* sAf, float* sOut, int tid) {
-  constexpr int BS = NN <= 4 ? 4 : 8;
+// Two copies of A sit block-diagonally in the 16x16 operand of v_mfma_f32_16x16x16bf16_1k, block
+// b on rows and K indices [8b, 8b + 8), so the 4 rows a lane holds (4g..4g+3, in both the K
+// operand and the result) lie in one block. A tile covers 16 columns per block (columns
+// col0 + 16 b), and waves take tiles round-robin. A and the lines are split into three bf16
+// pieces each; the five products of combined order 1 and 2 are accumulated in FP32, smallest
+// first, and the hi x hi products from zero, then added to them.
+// Lane layout (lane l, g = l/16, j = l%16): A[j][k = 4g + t], B[k = 4g + t][j], D[4g + r][j].
+template <int NN, int EPB, int NT>
+__device__ __forceinline__ void contract_mfma_bf16(const float* sF, const float* sAf, float* sOut, int tid) {
+  constexpr int BS = 8;
   constexpr int P = 16 / BS;
   constexpr int C = EPB * 6 * NN * NN;
   constexpr int TC = 16 * P;
   constexpr int T = (C + TC - 1) / TC;
   constexpr int W = NT / 64;
   static_assert(NT % 64 == 0, "whole waves");
-  static_assert(NN <= 8, "one block per 8 rows");
+  static_assert(NN > 4 && NN <= 8, "one block per 8 rows");
   const int lane = tid & 63, wave = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
   const int bj = j / BS, mj = j - bj * BS;    // block and line node of operand row j
   const int bd = (4 * g) / BS;                // block of the lane's result rows 4g..4g+3
   const int md = 4 * g - bd * BS;             // their first line node
-  if constexpr (CONTRACT == 1) {
-    float aop[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int bk = (4 * s) / BS, mk = 4 * s + g - bk * BS;
-      aop[s] = (bj == bk && mj < NN && mk < NN) ? sAf[mj * NN + mk] : 0.f;
-    }
-    for (int t = wave; t < T; t += W) {
-      // K step s reads block (4s)/BS for every lane; the results go to block bd, whose line is
-      // selected among the P mapped ones
-      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-      TileLine<NN> out(t * TC + j, C, sF);
-#pragma unroll
-      for (int b = 0; b < P; ++b) {
-        const TileLine<NN> ln = b == 0 ? out : TileLine<NN>(t * TC + 16 * b + j, C, sF);
-#pragma unroll
-        for (int s = b * BS / 4; s < (b + 1) * BS / 4; ++s)
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aop[s], ln.load(sF, 4 * s + g - b * BS), acc, 0, 0, 0);
-        if (b > 0) {
-          out.fo = bd == b ? ln.fo : out.fo;
-          out.oo = bd == b ? ln.oo : out.oo;
-          out.ls = bd == b ? ln.ls : out.ls;
-          out.ok = bd == b ? ln.ok : out.ok;
-          out.sh = bd == b ? ln.sh : out.sh;
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) out.store(sOut, md + r, acc[r]);
-    }
-  } else {
+  {
     // split-BF16 on v_mfma_f32_16x16x16bf16_1k (used at 8 nodes per line, where the two 8-row
     // blocks fill the tile): A and the lines are split into three bf16 pieces each; the five
     // products of combined order 1 and 2 are accumulated in FP32, smallest first, and the hi x hi
@@ -735,8 +748,9 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   const CT ir0 = (CT)mat[MAT_IRHO0];
   // the split-BF16 contraction's line descriptors (a small table every workgroup reads)
   constexpr bool kLines = CONTRACT == 2 && kLinesPerLane<NN>;
-  uint32_t ldesc[kLines ? LineShape<NN, EPB, NT>::GMAX : 1] = {};
-  if constexpr (kLines) line_descriptors<NN, EPB, NT>(tid, ldesc);
+  constexpr bool kLineDesc = kLines || CONTRACT == 1;
+  uint32_t ldesc[kLineDesc ? LineShape<NN, EPB, NT>::GMAX : 1] = {};
+  if constexpr (kLineDesc) line_descriptors<NN, EPB, NT>(tid, ldesc);
   double qq[3] = {0.0, 0.0, 0.0};
   CT dcoef = CT(0);
   if constexpr (LDG) {
@@ -966,7 +980,8 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
   }
   // the matrix-core contraction reads only phase 1's LDS (sF, sA) and writes its own array, so
   // it runs beside the flux pairs, before the barrier that phase 3 needs anyway
-  if constexpr (CONTRACT == 1 || (CONTRACT == 2 && !kLinesPerLane<NN>)) contract_mfma<NN, EPB, NT, CONTRACT>(sF, sA, sOut, tid);
+  if constexpr (CONTRACT == 2 && !kLinesPerLane<NN>) contract_mfma_bf16<NN, EPB, NT>(sF, sA, sOut, tid);
+  if constexpr (CONTRACT == 1) contract_lines_f32<NN, EPB, NT>(sF, sA, sOut, tid, ldesc);
   if constexpr (CONTRACT == 2 && kLinesPerLane<NN>) contract_lines_bf16<NN, EPB, NT>(sF, sm.sAp, sOut, tid, ldesc);
   __syncthreads();
 

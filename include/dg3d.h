@@ -50,7 +50,7 @@ extern "C" {
 
 /* derivative contraction used by the MIXED step */
 #define DG3D_CONTRACT_VALU 0      /* FP32 FMA on the vector ALUs */
-#define DG3D_CONTRACT_MFMA_F32 1  /* v_mfma_f32_16x16x4_f32 */
+#define DG3D_CONTRACT_MFMA_F32 1  /* v_mfma_f32_4x4x1_16b_f32, one line per lane */
 #define DG3D_CONTRACT_MFMA_BF16 2 /* split-BF16 (3 pieces per operand), FP32 accumulate */
 
 /* time schemes */

@@ -1,13 +1,15 @@
-"""The matrix-core contractions' mappings (contract_mfma and contract_lines_bf16 in
-3ddg_amd/csrc/dg3d_kernels.hip), emulated on the CPU for every stage shape the library builds.
+"""The matrix-core contractions' mappings (contract_mfma_bf16, contract_lines_bf16 and
+contract_lines_f32 in 3ddg_amd/csrc/dg3d_kernels.hip), emulated on the CPU for every stage shape
+the library builds.
 
 The emulation restates the kernel's index arithmetic: the column -> line map (column_line), the
-aligned diagonal blocks (one every 4 rows for n <= 4, every 8 rows for n >= 5), the lane layouts of
-v_mfma_f32_16x16x4f32 and v_mfma_f32_16x16x16bf16_1k, and the round-robin of tiles over waves.
-It checks what the kernel relies on without a bounds check of its own: every LDS read and write
-stays inside its array, every output line is written exactly once, and the tiled GEMM equals
-the direct per-node contraction out[i] = sum_m A[i][m] line[m] along each axis, where a P/rho0
-line enters shifted by its middle value (the FP32 stage's shifted pressure lines).
+line descriptors, the aligned diagonal blocks of the 16x16x16 tiles (one every 8 rows), the lane
+layouts of v_mfma_f32_16x16x16bf16_1k, v_mfma_f32_4x4x4bf16_1k and v_mfma_f32_4x4x1f32, and the
+round-robin of tiles and line groups over waves. It checks what the kernel relies on without a
+bounds check of its own: every LDS read and write stays inside its array, every output line is
+written exactly once, and the result equals the direct per-node contraction
+out[i] = sum_m A[i][m] line[m] along each axis, where a P/rho0 line enters shifted by its middle
+value (the FP32 stage's shifted pressure lines).
 """
 import numpy as np
 import pytest
@@ -34,11 +36,12 @@ def column_line(n, col):
     return le * 4 * NP + (0 if f == 0 else 1 + a) * NP + base, (le * 6 + af) * NP + base, ls
 
 
-def emulate(n, epb, mode, A, sF):
+def emulate(n, epb, A, sF):
+    """The 16x16x16 split-BF16 tiles (contract_mfma_bf16) in exact arithmetic."""
     NP, NF2 = n ** 3, n * n
     NT = (epb * elem_stride(n) + 63) // 64 * 64
     W = NT // 64
-    BS = 4 if n <= 4 else 8
+    BS = 8
     P = 16 // BS
     C = epb * 6 * NF2
     TC = 16 * P
@@ -73,23 +76,13 @@ def emulate(n, epb, mode, A, sF):
                 bj, mj = divmod(j, BS)
                 bd = (4 * g) // BS
                 md = 4 * g - bd * BS
-                if mode == 2:    # K rows 4g..4g+3 of the lane, one block
-                    ln = line(t * TC + 16 * bd + j)
-                    for q in range(4):
-                        k, mk = 4 * g + q, md + q
-                        Aop[j, k] = A[mj, mk] if (bj == bd and mj < n and mk < n) else 0.0
-                        Bop[k, j] = load(ln, mk)
-                    out.append((j, [(4 * g + r, ln, md + r) for r in range(4)]))
-                else:            # K step s covers rows 4s..4s+3, K index 4s + g
-                    lines = [line(t * TC + 16 * b + j) for b in range(P)]
-                    for s in range(4):
-                        kk = 4 * s + g
-                        bk = (4 * s) // BS
-                        mk = kk - bk * BS
-                        assert kk // BS == bk
-                        Aop[j, kk] = A[mj, mk] if (bj == bk and mj < n and mk < n) else 0.0
-                        Bop[kk, j] = load(lines[bk], mk)
-                    out.append((j, [(4 * g + r, lines[bd], md + r) for r in range(4)]))
+                # K rows 4g..4g+3 of the lane, one block
+                ln = line(t * TC + 16 * bd + j)
+                for q in range(4):
+                    k, mk = 4 * g + q, md + q
+                    Aop[j, k] = A[mj, mk] if (bj == bd and mj < n and mk < n) else 0.0
+                    Bop[k, j] = load(ln, mk)
+                out.append((j, [(4 * g + r, ln, md + r) for r in range(4)]))
             D = Aop @ Bop
             for j, rows in out:
                 for row, (ok, _, oo, ls, _sh), m in rows:
@@ -119,18 +112,17 @@ def direct(n, epb, A, sF):
     return ref
 
 
-@pytest.mark.parametrize("n", sorted(SHAPES))
+@pytest.mark.parametrize("n", [k for k in sorted(SHAPES) if k > 4])
 def test_tile_map_covers_every_line_once_in_bounds(n):
+    """The 16x16x16 split-BF16 tiles run at 8 nodes per line; their map holds from 5 up."""
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n))
     for epb in SHAPES[n]:
         sF = rng.standard_normal(epb * 4 * n ** 3)
         ref = direct(n, epb, A, sF)
-        # the 16x16x16 split-BF16 tiles run at 8 nodes per line only, but the map holds for all
-        for mode in (1, 2):
-            got, writes = emulate(n, epb, mode, A, sF)
-            assert (writes == 1).all(), (n, epb, mode, writes.min(), writes.max())
-            np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+        got, writes = emulate(n, epb, A, sF)
+        assert (writes == 1).all(), (n, epb, writes.min(), writes.max())
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
 
 
 def descriptor(n, epb, col):
@@ -216,5 +208,75 @@ def test_line_per_lane_map_covers_every_line_once_in_bounds(n):
         sF = rng.standard_normal(epb * 4 * n ** 3)
         ref = direct(n, epb, A, sF)
         got, writes = emulate_lines(n, epb, A, sF)
+        assert (writes == 1).all(), (n, epb, writes.min(), writes.max())
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def mfma_4x4x1(a_lanes, b_lanes):
+    """v_mfma_f32_4x4x1f32 as probed on the device: 16 blocks of 4 lanes; lane 4b + i holds row i
+    of block b's A (one K value), lane 4b + j column j of its B, and register r of lane 4b + j
+    receives row r, column j of block b's outer product."""
+    D = np.zeros((64, 4))
+    for b in range(16):
+        D[4 * b:4 * b + 4] = np.outer(b_lanes[4 * b:4 * b + 4], a_lanes[4 * b:4 * b + 4])
+    return D
+
+
+def emulate_lines_f32(n, epb, A, sF):
+    """The line-per-lane FP32 contraction (contract_lines_f32): the same line descriptors as the
+    split-BF16 one; each lane feeds row 4R + (lane & 3) of A's column k and its own line's node k,
+    one product per node and row block."""
+    NP, NF2 = n ** 3, n * n
+    NT = (epb * elem_stride(n) + 63) // 64 * 64
+    W = NT // 64
+    C = epb * 6 * NF2
+    G = (C + 63) // 64
+    RB = (n + 3) // 4
+    sOut = np.full(epb * 6 * NP, np.nan)
+    writes = np.zeros(epb * 6 * NP, int)
+    for wave in range(W):
+        for g in range(wave, G, W):
+            x = np.zeros((64, n))
+            dec = []
+            for lane in range(64):
+                d = descriptor(n, epb, g * 64 + lane)
+                ok = d >> 31 & 1
+                fo = d & 0xFFF if ok else 0
+                oo, ax, pr = d >> 12 & 0x1FFF, d >> 25 & 3, d >> 27 & 1
+                ls = NF2 if ax == 0 else (n if ax == 1 else 1)
+                for m in range(n):
+                    assert 0 <= fo + m * ls < sF.size
+                    x[lane, m] = sF[fo + m * ls]
+                sh = x[lane, n // 2] if pr else 0.0
+                x[lane] = (x[lane] - sh) if ok else 0.0
+                dec.append((ok, oo, ls))
+            for R in range(RB):
+                out = np.zeros((64, 4))
+                for k in range(n):
+                    a_lanes = np.array([A[4 * R + (lane & 3), k] if 4 * R + (lane & 3) < n else 0.0
+                                        for lane in range(64)])
+                    out += mfma_4x4x1(a_lanes, x[:, k])
+                for lane, (ok, oo, ls) in enumerate(dec):
+                    for r in range(4):
+                        row = 4 * R + r
+                        if ok and row < n:
+                            i = oo + row * ls
+                            assert 0 <= i < sOut.size
+                            sOut[i] = out[lane, r]
+                            writes[i] += 1
+    return sOut, writes
+
+
+@pytest.mark.parametrize("n", sorted(SHAPES))
+def test_fp32_line_per_lane_map_covers_every_line_once_in_bounds(n):
+    """The FP32 matrix-core contraction takes one line per lane at every order: every line is
+    written exactly once, every LDS index is in bounds, and the result equals the direct per-node
+    contraction."""
+    rng = np.random.default_rng(200 + n)
+    A = rng.standard_normal((n, n))
+    for epb in SHAPES[n]:
+        sF = rng.standard_normal(epb * 4 * n ** 3)
+        ref = direct(n, epb, A, sF)
+        got, writes = emulate_lines_f32(n, epb, A, sF)
         assert (writes == 1).all(), (n, epb, writes.min(), writes.max())
         np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())

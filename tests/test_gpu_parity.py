@@ -292,12 +292,31 @@ def test_mixed_matrix_core_contraction(contraction, order):
     assert errv.max() < 5e-6, errv
 
 
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6, 7])
+def test_fp32_matrix_core_contraction_equals_valu(order):
+    """The FP32 matrix-core contraction feeds each line's nodes to v_mfma_f32_4x4x1f32 in node
+    order, so it accumulates the same chain of correctly rounded multiply-adds as the VALU
+    contraction: the two mixed steps agree bit for bit."""
+    print(f"fp32_matrix_core_equals_valu order={order}", flush=True)
+    p = heterogeneous(small_c2(n_e=4, order=order, steps=20))
+    dt = problem.cfl_dt_host(p)
+    st = []
+    for con in (dg3d.CONTRACT_VALU, dg3d.CONTRACT_MFMA_F32):
+        ctx = problem.make_context(p, precision=dg3d.MIXED, contraction=con)
+        ctx.step(dg3d.LSRK54, dt, p.steps, 0.0)
+        st.append(ctx.get_state(0))
+        ctx.close()
+    assert np.linalg.norm(st[0]) > 0
+    assert np.array_equal(st[0], st[1]), np.abs(st[0] - st[1]).max()
+
+
 @pytest.mark.parametrize("contraction", [dg3d.CONTRACT_MFMA_F32, dg3d.CONTRACT_MFMA_BF16])
 @pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
 def test_matrix_core_variants(contraction, scheme):
-    """Each variant is deterministic (two runs agree bit for bit). Across variants the elements
-    per workgroup change which diagonal block of the 16x16 tile a line lands in, and with it the
-    order in which the matrix core sums the line's products, so variants agree to FP32 rounding."""
+    """Each variant is deterministic (two runs agree bit for bit), and the variants agree to FP32
+    rounding (with one line per lane, as here at 5 nodes per line, a line's sums do not depend on
+    the workgroup shape; the 16x16x16 split-BF16 tiles at 8 nodes per line place a line in one of
+    two diagonal blocks by the shape, which changes the order of the matrix core's sums)."""
     print(f"matrix_core_variants contraction={contraction} scheme={scheme}", flush=True)
     p = small_c2(n_e=5, order=4, steps=4)
     dt = problem.cfl_dt_host(p)
