@@ -642,13 +642,14 @@ def main():
         progress("256-solve batch done")
     if not args.no_extra and args.c5_steps > 0:
         # combined workload: stretched two-layer 48^3 N=5 with a focused burst; mixed with the
-        # split-BF16 matrix-core contraction against FP64, and the VALU contraction over the same
-        # trajectory beside it
+        # split-BF16 matrix-core contraction against FP64, and the FP32 matrix-core and VALU
+        # contractions over the same trajectory beside it
         out["combined_c5"] = compare_precisions(pk, pk.problem.config_c5(steps=args.c5_steps),
                                                 args.c5_steps - args.warmup, args.warmup,
                                                 mixed_contraction=pk.dg3d.CONTRACT_MFMA_BF16,
                                                 other_contractions=False,
-                                                full_length_others=(pk.dg3d.CONTRACT_VALU,))
+                                                full_length_others=(pk.dg3d.CONTRACT_MFMA_F32,
+                                                                    pk.dg3d.CONTRACT_VALU))
         progress("48^3 N=5 combined workload done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pk.problem.config_c2(num_elements=args.elements), args.cpu_steps)
