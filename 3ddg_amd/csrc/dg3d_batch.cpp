@@ -352,7 +352,7 @@ int dg3d_batch_create(int count, const dg3d_solve_desc* descs, int precision, dg
     }
     h.E = (int64_t)h.nE[0] * h.nE[1] * h.nE[2];
     h.fs = dg3d::field_slots(h.E, h.n);
-    if (h.fs >= (1ll << 29)) return bail(DG3D_E_INVALID, tag + "2^29 or more nodes in one solve");
+    if (h.fs >= (1ll << 29)) return bail(DG3D_E_INVALID, tag + "a field takes 2^29 or more node slots");
     h.slot0 = slot;
     slot += 4 * h.fs;
     h.dt = d.dt;

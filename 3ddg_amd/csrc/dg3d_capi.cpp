@@ -677,7 +677,8 @@ int dg3d_create(const dg3d_desc* d, dg3d_ctx** out) {
       dg3d::field_slots((int64_t)d->num_elements[0] * d->num_elements[1] * d->num_elements[2], d->order + 1) >=
           (1ll << 29)) {
     // the reductions and the probe index a register's state fields with 32-bit slot numbers
-    g_create_error = "2^29 or more nodes in one block: split the domain into blocks (NUM_BLOCKS)";
+    g_create_error = "a field of the block takes 2^29 or more node slots (the nodes with their padding and group "
+                     "gaps): split the domain into blocks (NUM_BLOCKS)";
     return DG3D_E_INVALID;
   }
   if (d->num_registers < 1 || d->num_registers > 64) { g_create_error = "num_registers must be 1..64"; return DG3D_E_INVALID; }

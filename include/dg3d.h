@@ -73,8 +73,11 @@ typedef struct dg3d_ctx dg3d_ctx;
 
 typedef struct dg3d_desc {
   int order;               /* N (1..7): n = N+1 LGL nodes per axis */
-  int num_elements[3];     /* interior elements per axis of the block (x, y, z); at most
-                              2^29 - 1 nodes (elements x (order+1)^3) per block */
+  int num_elements[3];     /* interior elements per axis of the block (x, y, z); a field of the
+                              block may take at most 2^29 - 1 node slots: the elements x (order+1)^3
+                              nodes, each element's rounded up to a multiple of 32 at 3, 5, 6 and 7
+                              nodes per line, and the gap at the end of each group of elements (up
+                              to 14 %) */
   int periodic[3];         /* BC_IS_PERIODIC (driver.cpp:170) */
   int precision;           /* DG3D_FP64 or DG3D_MIXED (for dg3d_step) */
   int contraction;         /* DG3D_CONTRACT_* (MIXED only) */

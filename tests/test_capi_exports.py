@@ -91,7 +91,7 @@ def test_workgroup_orders_are_permutations():
 
 
 def test_create_rejects_bad_descriptors_on_the_host():
-    """Descriptors the library cannot run are rejected before any device call: 2^29 nodes or more
+    """Descriptors the library cannot run are rejected before any device call: 2^29 node slots or more
     in one block (32-bit offsets within a field), a bad contraction or precision, a missing
     operator, an empty axis."""
     import numpy as np
@@ -119,7 +119,7 @@ def test_create_rejects_bad_descriptors_on_the_host():
         assert not h.value
         return rc, L.dg3d_create_error()
 
-    # 102^3 elements at N=7 take 512 slots each: 5.4e8 >= 2^29 nodes
+    # 102^3 elements at N=7 take 512 slots each: 5.4e8 >= 2^29 node slots
     rc, msg = create(desc(nE=(102, 102, 102)))
     assert rc == dg3d.E_INVALID and b"2^29" in msg
     rc, msg = create(desc(contraction=7))
