@@ -62,3 +62,39 @@ def small_c2(n_e=4, order=4, steps=60):
 
 def small_c5(n_e=4, steps=60):
     return problem.config_c5(num_elements=n_e, steps=steps)
+
+
+def discrete_mass(p, state):
+    """The discrete mass of rho': the sum over elements of the element volume times
+    sum_abc w_a w_b w_c rho'_abc. With the exact mass matrix, 1^T M is the vector of LGL weights,
+    so this is the integral of rho' that the weak form (S = D^T W, single-valued face fluxes)
+    conserves. Returns it and the same sum of |rho'| (its scale)."""
+    n = p.n
+    w = np.asarray(p.weights)
+    w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
+    rho = np.asarray(state).reshape(-1)[: p.E * n ** 3].reshape(p.E, n ** 3)
+    wx, wy, wz = (np.asarray(p.widths[a], float) for a in range(3))
+    vol = (wx[:, None, None] * wy[None, :, None] * wz[None, None, :]).reshape(-1)  # k fastest
+    return float(np.sum(vol * (rho @ w3))), float(np.sum(vol * (np.abs(rho) @ w3)))
+
+
+def periodic_mass_box(steps, uniform_rho0=True):
+    """8^3 periodic box without sponge around the c1 Gaussian pulse, with per-element c and B/A
+    (B/A = 0 on some elements); rho0 uniform unless asked otherwise."""
+    p = heterogeneous(small_c1(steps=steps))
+    if uniform_rho0:
+        p.rho0 = np.full(p.E, 1000.0)
+    p.periodic = (1, 1, 1)
+    p.sigma = np.zeros(p.E)
+    return p
+
+
+def plane_burst_column():
+    """The 64^3 N=4 workload's 1 MHz, 1e6 Pa tone burst (B/A = 5) in a column of 2 x 2 x 64
+    elements of 0.75 mm, periodic across (x, y) so that the wave is exactly plane, with the
+    8-element sponge at the far (+z) end only."""
+    h = 0.75e-3
+    p = problem._burst_problem("plane_column", 4, (2, 2, 64), h, 1)
+    p.periodic = (1, 1, 0)
+    problem.set_sponge(p, (0, 0, 0, 0, 0, 8), 3 * 1500.0 / (8 * h), ramp=1)
+    return p

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle_bind as ob
-from helpers import dg3d, problem, rel_l2, small_c1
+from helpers import dg3d, discrete_mass, periodic_mass_box, plane_burst_column, problem, rel_l2, small_c1
 
 
 # ---- per-line solve vs the reference's QR solve (Element.cpp:602, Matrix.h:289-445) ----
@@ -609,3 +609,68 @@ def test_oracle_boundary_source_plane_wave(N, tol):
     face = z == 0.0
     assert np.abs(out[0][face] * c * c - s(T)).max() / amp < tol
     assert np.abs(out[1:3]).max() < 1e-12 * np.abs(out[3]).max()   # no transverse motion
+
+
+# ---- system-level pins: discrete mass in a periodic box, plane-wave second harmonic ----
+
+def _oracle(p):
+    o = ob.Oracle(p, literal=False, num_registers=4)
+    ob.load_problem_into(o, p, 0)
+    if p.source.face >= 0:
+        s = p.source
+        o.set_source(s.face, s.amplitude, s.frequency, s.phase, s.t0, s.tau, s.lead)
+    o.set_time(0.0)
+    return o
+
+
+def test_oracle_mass_conserved_in_periodic_box():
+    """With rho0 uniform (c and B/A varying per element) the oracle's LSRK54 steps keep the discrete
+    mass to rounding: the mass flux of Element.cpp:541-600 is the same value seen from either side
+    and the weak-form volume term sums to zero. With rho0 varying, the lifted flux is scaled by
+    each side's own rho0 (Element.cpp:602, q = 1/rho0) and the mass moves at the percent level:
+    the check sees a flux that differs between the two sides of a face."""
+    p = periodic_mass_box(100)
+    o = _oracle(p)
+    dt = problem.cfl_dt_host(p)
+    m0, a0 = discrete_mass(p, o.get_state(0))
+    for _ in range(p.steps):
+        o.lsrk54(dt, 0, 1, 2)
+    m1, _ = discrete_mass(p, o.get_state(0))
+    assert abs(m1 - m0) < 1e-13 * a0, (m0, m1)
+    q = periodic_mass_box(30, uniform_rho0=False)
+    o = _oracle(q)
+    dt = problem.cfl_dt_host(q)
+    n0, b0 = discrete_mass(q, o.get_state(0))
+    for _ in range(q.steps):
+        o.lsrk54(dt, 0, 1, 2)
+    n1, _ = discrete_mass(q, o.get_state(0))
+    assert abs(n1 - n0) > 1e-4 * b0, (n0, n1)
+
+
+def test_oracle_plane_burst_second_harmonic_matches_fubini():
+    """The oracle's plane tone burst (2 x 2 x 64 column, periodic across (x, y), B/A = 5, N=4) at
+    the probe 2.25 mm deep: the Hann-windowed fundamental and second harmonic of the pressure
+    trace match the first-order Fubini solution with beta = 1 + B/(2A) to 3 % and 5 % (measured
+    0.984 and 1.022). Through beta, a missing 1/2 in the B/A term of the pressure
+    (kernelFuncs.cpp:17-20) would raise the harmonic by about 70 %, and dropping rho' from the
+    mass flux (kernelFuncs.cpp:26-30) would lower it by about 30 %."""
+    import bench
+    p = plane_burst_column()
+    dt = problem.cfl_dt_host(p)
+    pr = bench.probe_node(p)
+    depth = bench.probe_depth(p, pr)
+    steps = bench.burst_window_steps(p, dt, depth)
+    o = _oracle(p)
+    e = pr // p.n ** 3
+    c, r0, ba = float(p.c0[e]), float(p.rho0[e]), float(p.b_over_a[e])
+    tr = np.empty(steps)
+    for i in range(steps):
+        o.lsrk54(dt, 0, 1, 2)
+        r = o.get_state(0)[pr]
+        tr[i] = c * c * (r + 0.5 * ba / r0 * r * r)   # kernelFuncs.cpp:17-20 times rho0
+    f = p.source.frequency
+    h1, h2 = bench.dft_bin(tr, dt, f), bench.dft_bin(tr, dt, 2 * f)
+    e2, e1, sigma = bench.fubini_estimate(p, depth, dt, len(tr))
+    assert 0.005 < sigma < 0.05
+    assert abs(h1 / e1 - 1.0) < 0.03, (h1, e1)
+    assert abs(h2 / e2 - 1.0) < 0.05, (h2, e2, h2 / e2)
