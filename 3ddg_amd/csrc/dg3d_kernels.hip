@@ -96,7 +96,7 @@ __device__ __forceinline__ void elem_coords(const MeshArgs& m, uint32_t e, int* 
 
 // Neighbour element along `axis` on `side` given the element's coordinates, or -1 for the
 // exterior (halo) of the block.
-__device__ __forceinline__ int neighbour(const MeshArgs& m, int e, const int* ec, int axis, int side) {
+[[maybe_unused]] __device__ __forceinline__ int neighbour(const MeshArgs& m, int e, const int* ec, int axis, int side) {
   const int st = (int)m.stride[axis];
   const int pos = ec[axis];
   if (side == 0) {
@@ -111,7 +111,7 @@ __device__ __forceinline__ int neighbour(const MeshArgs& m, int e, const int* ec
 
 // Index of an element among the elements of a face normal to `axis` (the other two element
 // coordinates in (i, j, k) order).
-__device__ __forceinline__ int face_elem(const MeshArgs& m, const int* ec, int axis) {
+[[maybe_unused]] __device__ __forceinline__ int face_elem(const MeshArgs& m, const int* ec, int axis) {
   if (axis == 0) return ec[1] * m.nE[2] + ec[2];
   if (axis == 1) return ec[0] * m.nE[2] + ec[2];
   return ec[0] * m.nE[1] + ec[1];
@@ -223,7 +223,7 @@ __device__ __forceinline__ void face_coeffs(int side, CT c1, CT r1, CT ir1, CT c
 
 // A value known to be equal on every lane, as such to the compiler (bit-exact: the two halves
 // of the double are moved separately)
-__device__ __forceinline__ double wave_uniform(double v) {
+[[maybe_unused]] __device__ __forceinline__ double wave_uniform(double v) {
   const u32x2 u = __builtin_bit_cast(u32x2, v);
   u32x2 r;
   r[0] = __builtin_amdgcn_readfirstlane(u[0]);
@@ -236,7 +236,7 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 
 // x = hi + mid + lo as three bf16 pieces (hardware conversion, round to nearest even); both
 // residuals are exact in FP32 and the representation error is below 2^-26 |x|.
-__device__ __forceinline__ void split3(float x, short& h, short& m, short& l) {
+[[maybe_unused]] __device__ __forceinline__ void split3(float x, short& h, short& m, short& l) {
   const __bf16 a = (__bf16)x;
   const float r1 = x - (float)a;
   const __bf16 b = (__bf16)r1;
@@ -252,7 +252,7 @@ typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
 
 // split3 of two values at once: each conversion fills both halves of one register, in the
 // operand's order (element 0 in the low half), with the same bits as split3 per element.
-__device__ __forceinline__ void split3x2(float2v x, bf16x2v& h, bf16x2v& m, bf16x2v& l) {
+[[maybe_unused]] __device__ __forceinline__ void split3x2(float2v x, bf16x2v& h, bf16x2v& m, bf16x2v& l) {
   h = __builtin_convertvector(x, bf16x2v);
   const float2v r1 = x - __builtin_convertvector(h, float2v);
   m = __builtin_convertvector(r1, bf16x2v);
@@ -260,7 +260,7 @@ __device__ __forceinline__ void split3x2(float2v x, bf16x2v& h, bf16x2v& m, bf16
   l = __builtin_convertvector(r2, bf16x2v);
 }
 
-__device__ __forceinline__ bf16x4 join4(bf16x2v a, bf16x2v b) {
+[[maybe_unused]] __device__ __forceinline__ bf16x4 join4(bf16x2v a, bf16x2v b) {
   return __builtin_bit_cast(bf16x4, __builtin_shufflevector(a, b, 0, 1, 2, 3));
 }
 
@@ -367,7 +367,7 @@ __device__ __forceinline__ void line_descriptors(int tid, uint32_t* d) {
   }
 }
 
-__device__ __forceinline__ floatx4 mfma4(bf16x4 a, bf16x4 b, floatx4 c) {
+[[maybe_unused]] __device__ __forceinline__ floatx4 mfma4(bf16x4 a, bf16x4 b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(a, b, c, 0, 0, 0);
 }
 
@@ -1101,18 +1101,35 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
 // Waves per SIMD the register allocation must allow. The FP64 stage needs more than the 80
 // registers a target of 6 leaves when the RK register is loaded early, and with late loads at
 // every order but 2 and 5 nodes per line, and would spill to scratch (8-20 bytes per lane); those
-// shapes target 5 (96 registers), and so does the split-BF16 stage at 2 and 3 nodes per line
-// (20-28 bytes at 6). No kernel of the library uses scratch.
+// shapes target 5 (96 registers), and so do the matrix-core stages at 2 and 3 nodes per line
+// (split-BF16: 20-28 bytes at 6; FP32 with the RK33 epilogue: 20 bytes). No kernel of the
+// library uses scratch.
 template <int NN, typename CT, int MINW, bool LATE, int CONTRACT = 0>
 constexpr int waves_target() {
   constexpr bool fp64_heavy = sizeof(CT) == 8 && (!LATE || (NN != 2 && NN != 5));
-  constexpr bool bf16_heavy = CONTRACT == 2 && NN <= 3;
-  return (MINW > 5 && (fp64_heavy || bf16_heavy)) ? 5 : MINW;
+  constexpr bool mfma_heavy = CONTRACT != 0 && NN <= 3;
+  return (MINW > 5 && (fp64_heavy || mfma_heavy)) ? 5 : MINW;
+}
+
+// The waves per SIMD a shape's LDS leaves room for (160 KiB per CU, 4 SIMDs). The register target
+// asks for no more: at 2 nodes per line the FP64 flux coefficients and pairs take 160 bytes of LDS
+// per node, which holds those shapes to 3-4 waves per SIMD whatever their registers.
+template <int NN, int EPB, typename CT, int CONTRACT, bool LDG>
+constexpr int lds_waves() {
+  constexpr int wgs = (160 * 1024) / (int)sizeof(StageSmem<NN, EPB, CT, CONTRACT, LDG>);
+  constexpr int w = wgs * (Geo<NN, EPB>::NT / 64) / 4;
+  return w < 1 ? 1 : w;
+}
+template <int NN, int EPB, int MINW, typename CT, bool LATE, int CONTRACT, bool LDG>
+constexpr int stage_waves() {
+  constexpr int r = waves_target<NN, CT, MINW, LATE, CONTRACT>();
+  constexpr int l = lds_waves<NN, EPB, CT, CONTRACT, LDG>();
+  return r < l ? r : l;
 }
 
 template <int NN, int EPB, int MINW, typename CT, int EPI, typename DUT, int CONTRACT = 0, bool LATE = false,
           bool LDG = false>
-__global__ void __launch_bounds__((Geo<NN, EPB>::NT), (waves_target<NN, CT, MINW, LATE, CONTRACT>())) k_rhs(
+__global__ void __launch_bounds__((Geo<NN, EPB>::NT), (stage_waves<NN, EPB, MINW, CT, LATE, CONTRACT, LDG>())) k_rhs(
     const RhsLaunch L, const GroupMap gm) {
   __shared__ StageSmem<NN, EPB, CT, CONTRACT, LDG> sm;
   const uint32_t grp = map_group(blockIdx.x, gm);
