@@ -199,8 +199,13 @@ __device__ __forceinline__ void st_stream(T v, T* p) { __builtin_nontemporal_sto
 //   F1 = K4 mass_o + K5 mass_s + K6 mom_o + K7 mom_s
 // and the neighbour's kernel functions are mom_o = (rho + rho^2 K8) K9, mass_o = u (rho K11 + K10).
 // K12 = rho0_s / rho0_o converts a pressure shift expressed as P/rho0 of self into the
-// neighbour's P/rho0 (the FP32 stage's shifted lines, below).
+// neighbour's P/rho0 (the FP32 stage's shifted lines, below); the FP64 stage, which does not
+// shift its lines, keeps 12 coefficients per face in LDS.
 constexpr int kNK = 13;
+template <typename CT>
+constexpr int nk_of() {
+  return std::is_same<CT, double>::value ? 12 : 13;
+}
 template <typename CT>
 __device__ __forceinline__ void face_coeffs(int side, CT c1, CT r1, CT ir1, CT c2, CT r2, CT ba2, CT ir2, CT* K) {
   const CT z1 = r1 * c1, z2 = r2 * c2;
@@ -572,7 +577,7 @@ struct StageSmem {
   static constexpr int NP = NN * NN * NN, NF2 = NN * NN;
   CT sF[EPB * 4 * NP];                    // P/rho0, m_x, m_y, m_z per node
   CT sFl[EPB * 6 * NF2 * 2];              // scaled flux pair per face node
-  CT sK[EPB * 6 * kNK];                   // flux coefficients per face
+  CT sK[EPB * 6 * nk_of<CT>()];           // flux coefficients per face
   CT sIw[EPB * 3];                        // 1/width per element and axis
   float sOut[CONTRACT ? EPB * 6 * NP : 1];  // matrix-core contraction results
   // split-BF16 contraction: the pieces of A, [row block][K block][hi, mid, lo][row in block]
@@ -902,7 +907,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
       if ((hdir & 1) == 0) sIw[hle * 3 + 2 - (hdir >> 1)] = (CT)hiw;
     }
 #pragma unroll
-    for (int k = 0; k < kNK; ++k) sK[ht * kNK + k] = K[k];
+    for (int k = 0; k < nk_of<CT>(); ++k) sK[ht * nk_of<CT>() + k] = K[k];
   }
   if (ot >= 0 && ot < NN * NN) sA[ot] = av;
   if (ot >= 0 && ot < NN) {
@@ -935,7 +940,7 @@ __device__ __forceinline__ void stage_body(const RhsLaunch& L, const MeshArgs& m
     const CT* F = sF + fle * 4 * NP;
     const CT mom_s = F[sn];
     const CT mass_s = F[(1 + axis) * NP + sn];
-    const CT* K = sK + (fle * 6 + dir) * kNK;
+    const CT* K = sK + (fle * 6 + dir) * nk_of<CT>();
     const bool zero = (it & kItemExterior) != 0u;  // exterior face without a trace
     double vr = nrho[r], vu = nu[r];
     // FP32 compute: an opaque use keeps the conversions here; folded back to the loads, they
