@@ -292,18 +292,19 @@ def test_mixed_matrix_core_contraction(contraction, order):
     assert errv.max() < 5e-6, errv
 
 
+@pytest.mark.parametrize("scheme", [dg3d.LSRK54, dg3d.RK33])
 @pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6, 7])
-def test_fp32_matrix_core_contraction_equals_valu(order):
+def test_fp32_matrix_core_contraction_equals_valu(order, scheme):
     """The FP32 matrix-core contraction feeds each line's nodes to v_mfma_f32_4x4x1f32 in node
     order, so it accumulates the same chain of correctly rounded multiply-adds as the VALU
-    contraction: the two mixed steps agree bit for bit."""
-    print(f"fp32_matrix_core_equals_valu order={order}", flush=True)
+    contraction: the two mixed steps agree bit for bit, with either scheme."""
+    print(f"fp32_matrix_core_equals_valu order={order} scheme={scheme}", flush=True)
     p = heterogeneous(small_c2(n_e=4, order=order, steps=20))
     dt = problem.cfl_dt_host(p)
     st = []
     for con in (dg3d.CONTRACT_VALU, dg3d.CONTRACT_MFMA_F32):
         ctx = problem.make_context(p, precision=dg3d.MIXED, contraction=con)
-        ctx.step(dg3d.LSRK54, dt, p.steps, 0.0)
+        ctx.step(scheme, dt, p.steps, 0.0)
         st.append(ctx.get_state(0))
         ctx.close()
     assert np.linalg.norm(st[0]) > 0
