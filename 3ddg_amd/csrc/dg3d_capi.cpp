@@ -472,7 +472,7 @@ int reduce_partials(dg3d_ctx* c, int per_block, int blocks) {
 // Times every (kernel variant, workgroup order) pair on launches of a stage that reads and writes
 // the full register set (LSRK54 stage 2; RK33 stage 2 with register 0 as both inputs), after a
 // warm-up, in two sweeps of which each pair keeps the faster; the four fastest are timed again in
-// longer alternating sweeps, and the two best of those in alternating windows of about 0.3 s, of
+// longer alternating sweeps, and the three best of those in alternating windows of about 0.25 s, of
 // which the faster is kept. The launches write only the work
 // registers, which the next step's first stage overwrites without reading, so tuning leaves the
 // solution unchanged.
@@ -586,17 +586,20 @@ int autotune(dg3d_ctx* c, int scheme) {
       total[f] += ms;
     }
   }
-  // the two best of those in long alternating windows of about 0.3 s each: over short windows the
-  // 64^3 N=7 split-BF16 stage ranked workgroup order 0 first on some boxes, while every long run
-  // had order 2 ahead by 6 % (the chip's clocks settle over longer runs than a few launches)
+  // the three best of those in long alternating windows of about 0.25 s each: over short windows
+  // the 64^3 N=7 split-BF16 stage ranked workgroup order 0 first on some boxes, while every long
+  // run had order 2 ahead by 6 % (the chip's clocks settle over longer runs than a few launches);
+  // with only two in the windows, the 48^3 N=5 split-BF16 stage's sweeps of 16 launches often
+  // left out its best order, 2 % ahead in long runs on either data
   int order2[kFinal];
   for (int f = 0; f < kFinal; ++f) order2[f] = f;
   std::sort(order2, order2 + kFinal, [&](int a, int b) { return total[a] < total[b]; });
   const float launch_ms = total[order2[0]] / (kFinalSweeps * kFinalReps);
-  const int long_reps = std::max(16, std::min(512, (int)(300.f / std::max(launch_ms, 1e-3f))));
-  float window[2] = {0.f, 0.f};
+  const int long_reps = std::max(16, std::min(512, (int)(250.f / std::max(launch_ms, 1e-3f))));
+  constexpr int kWindows = 3;
+  float window[kWindows] = {0.f, 0.f, 0.f};
   for (int sweep = 0; sweep < 2; ++sweep) {
-    for (int f = 0; f < 2; ++f) {
+    for (int f = 0; f < kWindows; ++f) {
       const int fv = idx[order2[f]] / dg3d::kNumOrders, fx = idx[order2[f]] % dg3d::kNumOrders;
       e = dg3d::launch_rhs(tuning_launch(fv, fx), c->stream);
       if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
@@ -609,7 +612,10 @@ int autotune(dg3d_ctx* c, int scheme) {
       window[f] += ms;
     }
   }
-  const int k = idx[order2[window[1] < window[0] ? 1 : 0]];
+  int wbest = 0;
+  for (int f = 1; f < kWindows; ++f)
+    if (window[f] < window[wbest]) wbest = f;
+  const int k = idx[order2[wbest]];
   c->tune_variant[scheme] = k / dg3d::kNumOrders;
   c->tune_xcd[scheme] = k % dg3d::kNumOrders;
   // the pattern also filled the slots past the last element, which no stage writes
